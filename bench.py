@@ -1,0 +1,188 @@
+#!/usr/bin/env python3
+"""Benchmark: D3Q19 Couette flow on one MI355X, MLUPS and % of the HBM roofline
+per storage type.
+
+Workload (one "step" = one lattice Boltzmann time step of the whole domain):
+plane Couette 512^3 nodes, BGK, periodic x/y, bounce-back walls in z with a
+moving lid (pre_bounce_back + cavity_z_l U=(0.05,0,0) + post_bounce_back),
+tau = 0.8, synthetic initial state f_i = w_i (rho = 1, u = 0).  `value` is
+the FP64 drop-in's MLUPS (nodes x steps / s / 1e6, the definition of
+hippoLBM's log operator); the FP32/FP16/BF16 shifted-storage forms are
+reported beside it under "per_dtype".
+
+Timing: HIP events on the stream the step kernels run on, W untimed warm-up
+steps, then exactly K timed steps, device synchronised before and after.
+Roofline: algorithmic bytes per node update = 2*19*sizeof(storage) + 1
+(flag byte) against MI355X's 8 TB/s HBM peak.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2}
+
+
+def couette_domain(n, storage, collision="bgk"):
+    from hippolbm_amd.engine import Domain
+    dom = Domain((n, n, n), (True, True, False), 0.8, (0.0, 0.0, 0.0), storage=storage, collision=collision,
+                 dx=1.0)
+    dom.set_bcs([("pre_stream_bcs", "pre_bounce_back", {}), ("pre_stream_bcs", "cavity_z_l", {"U": (0.05, 0, 0)}),
+                 ("post_stream_bcs", "post_bounce_back", {})])
+    dom.init_obst()
+    dom.set_distribution(1.0)
+    return dom
+
+
+def time_steps(dom, steps, warmup):
+    from hippolbm_amd import hip
+    s = hip.Stream()
+    e0, e1 = hip.Event(), hip.Event()
+    dom.step(warmup, stream=s.handle)
+    hip.device_synchronize()
+    t0 = time.perf_counter()
+    e0.record(s)
+    dom.step(steps, stream=s.handle)
+    e1.record(s)
+    e1.synchronize()
+    hip.device_synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_ms(e1)
+    for o in (e0, e1, s):
+        o.close()
+    return ms, wall
+
+
+def measure(n, storage, steps, warmup, collision="bgk"):
+    dom = couette_domain(n, storage, collision)
+    ms, wall = time_steps(dom, steps, warmup)
+    # sanity: the run stayed finite and mass-conserving
+    dom.macro()
+    st = dom.stats()
+    dom.close()
+    nodes = n ** 3
+    ms_step = ms / steps
+    mlups = nodes * steps / (ms * 1e-3) / 1e6
+    bpn = 2 * 19 * BYTES[storage] + 1
+    gbs = nodes * bpn / (ms_step * 1e-3) / 1e9
+    return {"mlups": mlups, "ms_per_step": ms_step, "wall_s": wall, "bytes_per_node": bpn, "achieved_gbs": gbs,
+            "frac": gbs / HBM_PEAK_GBS, "sum_density": st["sum_density"], "max_u": st["max_velocity_norm"]}
+
+
+def copy_bandwidth():
+    import ctypes
+    from hippolbm_amd import _lib
+    ms = ctypes.c_double()
+    nbytes = 4 << 30
+    _lib.check("hlbm_copy_bench", _lib.lib().hlbm_copy_bench(nbytes, 20, None, ctypes.byref(ms)))
+    return 2 * nbytes / (ms.value * 1e-3) / 1e9
+
+
+def cpu_baseline(n=128, budget_s=12.0, max_steps=400):
+    """The FP64 oracle (a CPU restatement of the reference step chain, OpenMP
+    static) on the same Couette workload at n^3, timed on this host."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    o = orc.Oracle((n, n, n), (True, True, False), 1.0)
+    o.init_obst()
+    o.set_distribution(1.0)
+    o.macro_variables()
+
+    def step():
+        o.macro_variables()
+        o.bgk(0.8)
+        o.pre_bounce_back()
+        o.cavity(1, (0.05, 0.0, 0.0))
+        o.streaming()
+        o.post_bounce_back()
+
+    step()
+    steps = 0
+    t0 = time.perf_counter()
+    while steps < max_steps and time.perf_counter() - t0 < budget_s:
+        step()
+        steps += 1
+    dt = time.perf_counter() - t0
+    return {"value": n ** 3 * steps / dt / 1e6, "unit": "MLUPS", "cores": orc.lib().orc_threads(), "kind": "port",
+            "sample": f"oracle FP64 Couette {n}^3 BGK, {steps} steps ({dt:.1f} s), OpenMP static"}
+
+
+def read_traffic(name):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
+    p = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(p):
+        return None
+    vals = {}
+    for line in open(p):
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=int(os.environ.get("WORLD_SIZE", "1")))
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--dtypes", default="f64,f32,f16,bf16")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--only", default=None, help="run a single storage type (profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+              "(the domain-decomposed multi-GPU path is a design note, not code)", file=sys.stderr)
+        return 2
+    if args.only:
+        r = measure(args.size, args.only, args.steps, args.warmup)
+        print(json.dumps({"storage": args.only, **r}), flush=True)
+        return 0
+    n = args.size
+    per = {}
+    for dt in [d for d in args.dtypes.split(",") if d]:
+        per[dt] = measure(n, dt, args.steps, args.warmup)
+        print(f"# {dt}: {per[dt]['mlups']:.0f} MLUPS, {per[dt]['achieved_gbs']:.0f} GB/s "
+              f"({100 * per[dt]['frac']:.1f}% of 8 TB/s)", file=sys.stderr, flush=True)
+    main_r = per.get("f64") or next(iter(per.values()))
+    copy_gbs = copy_bandwidth()
+    cpu = None if args.no_cpu else cpu_baseline()
+    line = {
+        "metric": "MLUPS (D3Q19 BGK FP64 Couette 512^3, one MI355X) + % HBM roofline per storage dtype",
+        "value": main_r["mlups"],
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_r["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (f_i = w_i, rho = 1)",
+        "config": {"workload": f"couette_{n}^3_bgk_f64", "nodes": [n, n, n], "periodic": [True, True, False],
+                   "tau": 0.8, "bcs": "pre_bounce_back + cavity_z_l(U=0.05) + post_bounce_back",
+                   "pattern": "AB pull, fused collide+stream"},
+        "roofline": {"bound": "hbm", "achieved": main_r["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": main_r["frac"], "traffic": read_traffic(f"traffic_couette{n}_f64.txt"),
+                     "bytes_per_node": main_r["bytes_per_node"], "copy_kernel_gbs": copy_gbs},
+        "per_dtype": {k: {"mlups": v["mlups"], "frac": v["frac"], "achieved_gbs": v["achieved_gbs"],
+                          "bytes_per_node": v["bytes_per_node"], "ms_per_step": v["ms_per_step"]}
+                      for k, v in per.items()},
+        "latency": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,643 @@
+// api.hip -- the C ABI of include/hlbm.h.
+//
+// Host code only (kernels live in step_*.hip and util_kernels.hip).  Compiled
+// with -ffp-contract=off: the few FP64 constants prepared here (c_i.F, the
+// cavity coefficients, bounding boxes) must round as in hippoLBM.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hlbm.h"
+#include "launch.hpp"
+#include "util_kernels.hpp"
+
+using namespace hlbm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                        \
+  do {                                                                                       \
+    hipError_t e_ = (expr);                                                                  \
+    if (e_ != hipSuccess) return fail(HLBM_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+enum Phase { PHASE_PRE = 0, PHASE_POST = 1 };
+
+size_t elem_size(int storage) { return storage == HLBM_F64 ? 8 : (storage == HLBM_F32 ? 4 : 2); }
+
+}  // namespace
+
+struct hlbm_domain {
+  hlbm_domain_desc d;
+  int nx = 0, ny = 0, nz = 0;
+  uint32_t N = 0;
+  uint64_t stride = 0;
+  size_t esize = 8;
+  void* buf[2] = {nullptr, nullptr};
+  int cur = 0;
+  int phase = PHASE_PRE;
+  int64_t t = 0;
+  uint8_t* flags = nullptr;
+  double* m0 = nullptr;
+  double* m1 = nullptr;
+  double* red_part = nullptr;
+  double* red_out = nullptr;
+  double* tmp = nullptr;  // one FP64 population plane, conversions
+  bool any_wall = false;
+  StepParams plan{};
+  dim3 block{64, 4, 1};
+  int device = 0;
+  uint64_t bytes = 0;
+};
+
+static dim3 grid_for(const hlbm_domain* h) {
+  return dim3((h->nx + h->block.x - 1) / h->block.x, (h->ny + h->block.y - 1) / h->block.y, h->nz);
+}
+
+static StepParams base_params(hlbm_domain* h) {
+  StepParams p = h->plan;
+  p.nx = h->nx;
+  p.ny = h->ny;
+  p.nz = h->nz;
+  for (int d = 0; d < 3; d++) p.per[d] = h->d.periodic[d] ? 1 : 0;
+  p.N = h->N;
+  p.stride = h->stride;
+  p.flags = h->flags;
+  p.m0 = h->m0;
+  p.m1 = h->m1;
+  p.tau = h->d.tau;
+  for (int d = 0; d < 3; d++) p.F[d] = h->d.fext[d];
+  // c_i . F in the reference's order: ex*Fx + ey*Fy + ez*Fz (bgk.hpp:57, mrt.hpp:188)
+  for (int i = 0; i < Q; i++) p.ef[i] = CX(i) * h->d.fext[0] + CY(i) * h->d.fext[1] + CZ(i) * h->d.fext[2];
+  p.mrt_custom = h->d.mrt_custom;
+  for (int k = 0; k < 10; k++) p.mrt[k] = h->d.mrt_rates[k];
+  if (!h->any_wall) p.wall_bb = 0;
+  return p;
+}
+
+static int launch(hlbm_domain* h, int mode, bool write_macro, hipStream_t s) {
+  StepParams p = base_params(h);
+  p.fin = h->buf[h->cur];
+  p.fout = h->buf[h->cur ^ 1];
+  p.write_macro = write_macro ? 1 : 0;
+  hipError_t e;
+  const int coll = h->d.collision == HLBM_MRT ? COLL_MRT : COLL_BGK;
+  if (h->d.storage == HLBM_F64)
+    e = launch_step_f64(coll, mode, grid_for(h), h->block, p, s);
+  else
+    e = launch_step_lowp(h->d.storage, coll, mode, grid_for(h), h->block, p, s);
+  if (e != hipSuccess) return fail(HLBM_ERR_HIP, std::string("step kernel launch: ") + hipGetErrorString(e));
+  h->cur ^= 1;
+  return HLBM_OK;
+}
+
+// Bring the state to the reference's post-stream/post-BC form f^T.
+static int materialize(hlbm_domain* h, hipStream_t s) {
+  if (h->phase == PHASE_POST) {
+    int rc = launch(h, MODE_STREAM, false, s);
+    if (rc) return rc;
+    h->phase = PHASE_PRE;
+  }
+  return HLBM_OK;
+}
+
+extern "C" {
+
+const char* hlbm_last_error(void) { return g_err.c_str(); }
+
+const char* hlbm_version(void) { return "hippolbm-mi355x 0.1 (gfx950)"; }
+
+int hlbm_device_count(int32_t* count) {
+  if (!count) return fail(HLBM_ERR_ARG, "count is NULL");
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return HLBM_OK;
+}
+
+int hlbm_domain_create(const hlbm_domain_desc* desc, hlbm_domain** out) {
+  if (!desc || !out) return fail(HLBM_ERR_ARG, "desc/out is NULL");
+  *out = nullptr;
+  for (int d = 0; d < 3; d++)
+    if (desc->nodes[d] < 2) return fail(HLBM_ERR_ARG, "nodes must be >= 2 on every axis");
+  const uint64_t N64 = (uint64_t)desc->nodes[0] * desc->nodes[1] * desc->nodes[2];
+  if (N64 >= (1ull << 31)) return fail(HLBM_ERR_ARG, "domain too large for one subdomain (>= 2^31 nodes)");
+  if (desc->storage < HLBM_F64 || desc->storage > HLBM_BF16) return fail(HLBM_ERR_ARG, "bad storage");
+  if (desc->collision != HLBM_BGK && desc->collision != HLBM_MRT) return fail(HLBM_ERR_ARG, "bad collision");
+  if (desc->pattern != HLBM_PATTERN_AB) return fail(HLBM_ERR_UNSUPPORTED, "only the A/B pattern is built");
+  if (!(desc->tau > 0.0)) return fail(HLBM_ERR_ARG, "tau must be > 0");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(HLBM_ERR_NODEVICE, "no HIP device visible");
+
+  hlbm_domain* h = new hlbm_domain();
+  h->d = *desc;
+  h->nx = desc->nodes[0];
+  h->ny = desc->nodes[1];
+  h->nz = desc->nodes[2];
+  h->N = (uint32_t)N64;
+  h->esize = elem_size(desc->storage);
+  const uint64_t align = 256 / h->esize;
+  h->stride = (N64 + align - 1) / align * align;
+  (void)hipGetDevice(&h->device);
+  const char* blk = getenv("HLBM_BLOCK");
+  if (blk) {
+    int bx = 0, by = 0;
+    if (sscanf(blk, "%dx%d", &bx, &by) == 2 && bx > 0 && by > 0 && bx * by <= 256 && (bx * by) % 64 == 0)
+      h->block = dim3(bx, by, 1);
+  }
+  const size_t fbytes = (size_t)h->stride * Q * h->esize;
+  auto cleanup = [&](const std::string& what) {
+    for (void* p : {h->buf[0], h->buf[1], (void*)h->flags, (void*)h->m0, (void*)h->m1, (void*)h->red_part,
+                    (void*)h->red_out, (void*)h->tmp})
+      if (p) (void)hipFree(p);
+    delete h;
+    return fail(HLBM_ERR_NOMEM, "device allocation failed: " + what);
+  };
+  if (hipMalloc(&h->buf[0], fbytes) != hipSuccess) return cleanup("populations A");
+  if (hipMalloc(&h->buf[1], fbytes) != hipSuccess) return cleanup("populations B");
+  if (hipMalloc((void**)&h->flags, N64) != hipSuccess) return cleanup("flags");
+  if (hipMalloc((void**)&h->m0, N64 * 8) != hipSuccess) return cleanup("rho");
+  if (hipMalloc((void**)&h->m1, N64 * 24) != hipSuccess) return cleanup("u");
+  if (hipMalloc((void**)&h->red_part, 7 * 8 * 1024) != hipSuccess) return cleanup("reduction");
+  if (hipMalloc((void**)&h->red_out, 7 * 8) != hipSuccess) return cleanup("reduction");
+  h->bytes = 2 * fbytes + N64 * (1 + 8 + 24) + 7 * 8 * 1025;
+  if (hipMemset(h->buf[0], 0, fbytes) != hipSuccess || hipMemset(h->buf[1], 0, fbytes) != hipSuccess ||
+      hipMemset(h->flags, 0, N64) != hipSuccess || hipMemset(h->m0, 0, N64 * 8) != hipSuccess ||
+      hipMemset(h->m1, 0, N64 * 24) != hipSuccess)
+    return cleanup("memset");
+  if (hipDeviceSynchronize() != hipSuccess) return cleanup("sync");
+  *out = h;
+  // zero populations in the storage's own encoding (shifted storage holds -w_i)
+  if (h->d.storage != HLBM_F64) {
+    int lo[3] = {0, 0, 0}, hi[3] = {h->nx - 1, h->ny - 1, h->nz - 1};
+    if (launch_set_distribution(h->d.storage, h->buf[0], h->stride, h->nx, h->ny, lo, hi, 0.0, 0) != hipSuccess ||
+        launch_set_distribution(h->d.storage, h->buf[1], h->stride, h->nx, h->ny, lo, hi, 0.0, 0) != hipSuccess ||
+        hipDeviceSynchronize() != hipSuccess) {
+      hlbm_domain_destroy(h);
+      *out = nullptr;
+      return fail(HLBM_ERR_HIP, "initial fill failed");
+    }
+  }
+  return HLBM_OK;
+}
+
+int hlbm_domain_destroy(hlbm_domain* h) {
+  if (!h) return HLBM_OK;
+  (void)hipDeviceSynchronize();
+  for (void* p : {h->buf[0], h->buf[1], (void*)h->flags, (void*)h->m0, (void*)h->m1, (void*)h->red_part,
+                  (void*)h->red_out, (void*)h->tmp})
+    if (p) (void)hipFree(p);
+  delete h;
+  return HLBM_OK;
+}
+
+int hlbm_domain_bytes(const hlbm_domain* h, uint64_t* bytes) {
+  if (!h || !bytes) return fail(HLBM_ERR_ARG, "NULL argument");
+  *bytes = h->bytes + (h->tmp ? (uint64_t)h->N * 8 : 0);
+  return HLBM_OK;
+}
+
+int hlbm_set_params(hlbm_domain* h, double tau, const double fext[3]) {
+  if (!h || !fext) return fail(HLBM_ERR_ARG, "NULL argument");
+  if (!(tau > 0.0)) return fail(HLBM_ERR_ARG, "tau must be > 0");
+  h->d.tau = tau;
+  for (int d = 0; d < 3; d++) h->d.fext[d] = fext[d];
+  return HLBM_OK;
+}
+
+// Compile the three operator lists into the kernel's boundary plan.
+int hlbm_set_bcs(hlbm_domain* h, const hlbm_bc* list, int32_t n) {
+  if (!h || (n > 0 && !list) || n < 0) return fail(HLBM_ERR_ARG, "bad bc list");
+  StepParams P{};
+  bool pre_bb = false, post_bb = false;
+  bool cav_after_pre[2] = {false, false};
+  double cav_u[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  int last_stage = 0;
+  for (int k = 0; k < n; k++) {
+    const hlbm_bc& b = list[k];
+    if (b.stage < last_stage) return fail(HLBM_ERR_ARG, "bc list must be ordered pre_stream, post_stream, boundary");
+    last_stage = b.stage;
+    if (b.stage == HLBM_STAGE_PRE_STREAM) {
+      switch (b.kind) {
+        case HLBM_BC_PRE_BOUNCE_BACK:
+          pre_bb = true;
+          break;
+        case HLBM_BC_WALL_BOUNCE_BACK:
+          P.wall_bb = 1;
+          break;
+        case HLBM_BC_CAVITY_Z_0:
+        case HLBM_BC_CAVITY_Z_L: {
+          const int side = b.kind == HLBM_BC_CAVITY_Z_L ? 1 : 0;
+          // a cavity before pre_bounce_back acts on buffers that pre_bounce_back then overwrites
+          if (!pre_bb) break;
+          if (cav_after_pre[side]) return fail(HLBM_ERR_UNSUPPORTED, "more than one cavity operator on one z face");
+          cav_after_pre[side] = true;
+          for (int d = 0; d < 3; d++) cav_u[side][d] = b.U[d];
+          break;
+        }
+        default:
+          return fail(HLBM_ERR_UNSUPPORTED, "only pre_bounce_back, wall_bounce_back and cavity_z_* may run before streaming");
+      }
+    } else if (b.stage == HLBM_STAGE_POST_STREAM || b.stage == HLBM_STAGE_BOUNDARY) {
+      switch (b.kind) {
+        case HLBM_BC_POST_BOUNCE_BACK:
+          post_bb = true;
+          // The gather already applied on-node bounce-back (+ moving wall); a
+          // post_bounce_back re-applies it only after an operator changed the node.
+          if (pre_bb && P.nprog > 0) {
+            if (P.nprog >= MAX_PROG) return fail(HLBM_ERR_UNSUPPORTED, "too many boundary operators");
+            P.prog[P.nprog].kind = PK_REBB;
+            P.prog[P.nprog].plane = -1;
+            P.nprog++;
+          }
+          break;
+        case HLBM_BC_LID_DRIVEN_CAVITY:
+        case HLBM_BC_NEUMANN:
+          if (b.plane < 0 || b.plane > 5) return fail(HLBM_ERR_ARG, "bad plane");
+          if (P.nprog >= MAX_PROG) return fail(HLBM_ERR_UNSUPPORTED, "too many boundary operators");
+          P.prog[P.nprog].kind = b.kind == HLBM_BC_LID_DRIVEN_CAVITY ? PK_LID : PK_NEUMANN;
+          P.prog[P.nprog].plane = b.plane;
+          for (int d = 0; d < 3; d++) P.prog[P.nprog].U[d] = b.U[d];
+          P.prog_planes |= 1 << b.plane;
+          P.nprog++;
+          break;
+        default:
+          return fail(HLBM_ERR_UNSUPPORTED, "only post_bounce_back, lid_driven_cavity and neumann may run after streaming");
+      }
+    } else {
+      return fail(HLBM_ERR_ARG, "bad stage");
+    }
+  }
+  P.plane_bb = (pre_bb && post_bb) ? 1 : 0;
+  // the moving-wall term exists only through the saved/restored buffers of a non-periodic z axis
+  const double L = h->nz;  // domain_size_ z (cavity.cu:69, cavity.hpp:58-62)
+  for (int side = 0; side < 2; side++) {
+    P.cav_on[side] = (P.plane_bb && cav_after_pre[side] && !h->d.periodic[2]) ? 1 : 0;
+    const double uxx = cav_u[side][0] * (1 + 0.5 / (L - 1));
+    const double uyy = cav_u[side][1] * (1 + 0.5 / (L - 1));
+    const double uzz = cav_u[side][2] * (1 + 0.5 / (L - 1));
+    for (int j = 0; j < 5; j++) {
+      const int q = Faces::in[4 + side][j];
+      P.cav[side][j] = 6. * WD(q) * (CX(q) * uxx + CY(q) * uyy + CZ(q) * uzz);
+    }
+  }
+  bool has_rebb = false;
+  for (int k = 0; k < P.nprog; k++) has_rebb |= P.prog[k].kind == PK_REBB;
+  if (has_rebb) {
+    for (int dim = 0; dim < 3; dim++)
+      if (!h->d.periodic[dim]) P.prog_planes |= dim == 2 ? 3 : (dim == 1 ? 12 : 48);
+  }
+  h->plan = P;
+  return HLBM_OK;
+}
+
+int hlbm_init_obst(hlbm_domain* h) {
+  if (!h) return fail(HLBM_ERR_ARG, "NULL domain");
+  HIP_TRY(hipMemset(h->flags, 0, h->N));
+  HIP_TRY(hipDeviceSynchronize());
+  h->any_wall = false;
+  return HLBM_OK;
+}
+
+static int refresh_any_wall(hlbm_domain* h) {
+  std::vector<uint8_t> fl(h->N);
+  HIP_TRY(hipMemcpy(fl.data(), h->flags, h->N, hipMemcpyDeviceToHost));
+  h->any_wall = std::any_of(fl.begin(), fl.end(), [](uint8_t v) { return v != 0; });
+  return HLBM_OK;
+}
+
+int hlbm_add_obstacles(hlbm_domain* h, const hlbm_obstacle* list, int32_t n) {
+  if (!h || (n > 0 && !list) || n < 0) return fail(HLBM_ERR_ARG, "bad obstacle list");
+  if (n == 0) return HLBM_OK;
+  if (!(h->d.dx > 0.0)) return fail(HLBM_ERR_ARG, "desc.dx must be > 0 to rasterize obstacles");
+  const double dx = h->d.dx;
+  const int nn[3] = {h->nx, h->ny, h->nz};
+  std::vector<ObstacleDev> obs;
+  for (int k = 0; k < n; k++) {
+    const hlbm_obstacle& o = list[k];
+    if (o.kind != 0 && o.kind != 1) return fail(HLBM_ERR_ARG, "obstacle kind must be 0 (ball) or 1 (wall)");
+    double bmin[3], bmax[3];
+    for (int d = 0; d < 3; d++) {
+      bmin[d] = o.kind == 0 ? o.p0[d] - o.radius : o.p0[d];
+      bmax[d] = o.kind == 0 ? o.p0[d] + o.radius : o.p1[d];
+    }
+    ObstacleDev od{};
+    od.kind = o.kind;
+    bool inside = true;
+    for (int d = 0; d < 3; d++) {
+      // int(b/dx) truncation (update_obstacles.cu:61-62); global node index == our node index
+      const int a = int(bmin[d] / dx), b = int(bmax[d] / dx);
+      if (b < 0 || a > nn[d] - 1) inside = false;
+      od.lo[d] = std::max(a, 0);
+      od.hi[d] = std::min(b, nn[d] - 1);
+      od.p0[d] = o.p0[d];
+      od.p1[d] = o.p1[d];
+    }
+    od.r2 = o.radius * o.radius;
+    if (inside) obs.push_back(od);
+  }
+  if (obs.empty()) return HLBM_OK;
+  ObstacleDev* dobs = nullptr;
+  HIP_TRY(hipMalloc(&dobs, obs.size() * sizeof(ObstacleDev)));
+  hipError_t e = hipMemcpy(dobs, obs.data(), obs.size() * sizeof(ObstacleDev), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_rasterize(h->flags, h->nx, h->ny, h->nz, dx, dobs, (int)obs.size(), 0);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  (void)hipFree(dobs);
+  if (e != hipSuccess) return fail(HLBM_ERR_HIP, std::string("rasterize: ") + hipGetErrorString(e));
+  return refresh_any_wall(h);
+}
+
+// reference layout <-> node layout index helpers
+static inline size_t ref_index(const hlbm_domain* h, int x, int y, int z) {
+  const size_t Lx = h->nx + 4, Ly = h->ny + 4;
+  return ((size_t)(z + 2) * Ly + (y + 2)) * Lx + (x + 2);
+}
+
+static size_t ref_count(const hlbm_domain* h) { return (size_t)(h->nx + 4) * (h->ny + 4) * (h->nz + 4); }
+
+int hlbm_set_flags(hlbm_domain* h, const int32_t* obst, int32_t layout) {
+  if (!h || !obst) return fail(HLBM_ERR_ARG, "NULL argument");
+  std::vector<uint8_t> fl(h->N);
+  size_t k = 0;
+  for (int z = 0; z < h->nz; z++)
+    for (int y = 0; y < h->ny; y++)
+      for (int x = 0; x < h->nx; x++, k++) {
+        const int32_t v = layout == HLBM_LAYOUT_REF ? obst[ref_index(h, x, y, z)] : obst[k];
+        if (v != -1 && v != -2) return fail(HLBM_ERR_ARG, "obstacle values must be -1 (fluid) or -2 (wall)");
+        fl[k] = v == -2 ? 1 : 0;
+      }
+  HIP_TRY(hipMemcpy(h->flags, fl.data(), h->N, hipMemcpyHostToDevice));
+  h->any_wall = std::any_of(fl.begin(), fl.end(), [](uint8_t v) { return v != 0; });
+  return HLBM_OK;
+}
+
+int hlbm_get_flags(hlbm_domain* h, int32_t* obst, int32_t layout) {
+  if (!h || !obst) return fail(HLBM_ERR_ARG, "NULL argument");
+  std::vector<uint8_t> fl(h->N);
+  HIP_TRY(hipMemcpy(fl.data(), h->flags, h->N, hipMemcpyDeviceToHost));
+  size_t k = 0;
+  for (int z = 0; z < h->nz; z++)
+    for (int y = 0; y < h->ny; y++)
+      for (int x = 0; x < h->nx; x++, k++) {
+        const int32_t v = fl[k] ? -2 : -1;
+        if (layout == HLBM_LAYOUT_REF)
+          obst[ref_index(h, x, y, z)] = v;
+        else
+          obst[k] = v;
+      }
+  return HLBM_OK;
+}
+
+int hlbm_set_distribution(hlbm_domain* h, double value, const double* bounds) {
+  if (!h) return fail(HLBM_ERR_ARG, "NULL domain");
+  int rc = materialize(h, 0);
+  if (rc) return rc;
+  int lo[3] = {0, 0, 0}, hi[3] = {h->nx - 1, h->ny - 1, h->nz - 1};
+  if (bounds) {
+    if (!(h->d.dx > 0.0)) return fail(HLBM_ERR_ARG, "desc.dx must be > 0 for a bounded set_distribution");
+    const int nn[3] = {h->nx, h->ny, h->nz};
+    for (int d = 0; d < 3; d++) {
+      const int a = int(bounds[d] / h->d.dx), b = int(bounds[3 + d] / h->d.dx);
+      if (b < 0 || a > nn[d] - 1) return HLBM_OK;  // no intersection (set_distribution.cu:94-95)
+      lo[d] = std::max(a, 0);
+      hi[d] = std::min(b, nn[d] - 1);
+    }
+  }
+  HIP_TRY(launch_set_distribution(h->d.storage, h->buf[h->cur], h->stride, h->nx, h->ny, lo, hi, value, 0));
+  HIP_TRY(hipDeviceSynchronize());
+  return HLBM_OK;
+}
+
+static int ensure_tmp(hlbm_domain* h) {
+  if (!h->tmp) HIP_TRY(hipMalloc((void**)&h->tmp, (size_t)h->N * 8));
+  return HLBM_OK;
+}
+
+int hlbm_set_populations_f64(hlbm_domain* h, const double* f, int32_t layout) {
+  if (!h || !f) return fail(HLBM_ERR_ARG, "NULL argument");
+  int rc = ensure_tmp(h);
+  if (rc) return rc;
+  std::vector<double> plane;
+  const size_t Nall = ref_count(h);
+  char* base = (char*)h->buf[h->cur];
+  for (int i = 0; i < Q; i++) {
+    const double* src = f + (size_t)i * h->N;
+    if (layout == HLBM_LAYOUT_REF) {
+      plane.resize(h->N);
+      size_t k = 0;
+      for (int z = 0; z < h->nz; z++)
+        for (int y = 0; y < h->ny; y++)
+          for (int x = 0; x < h->nx; x++, k++) plane[k] = f[(size_t)i * Nall + ref_index(h, x, y, z)];
+      src = plane.data();
+    }
+    void* dst = base + (size_t)i * h->stride * h->esize;
+    if (h->d.storage == HLBM_F64) {
+      HIP_TRY(hipMemcpy(dst, src, (size_t)h->N * 8, hipMemcpyHostToDevice));
+    } else {
+      HIP_TRY(hipMemcpy(h->tmp, src, (size_t)h->N * 8, hipMemcpyHostToDevice));
+      HIP_TRY(launch_convert(h->d.storage, true, dst, h->tmp, h->N, i, 0));
+    }
+  }
+  HIP_TRY(hipDeviceSynchronize());
+  h->phase = PHASE_PRE;
+  return HLBM_OK;
+}
+
+static void fill_periodic_ghosts(const hlbm_domain* h, double* fref, int ncomp) {
+  // single-rank update_ghost: every periodic shift copies the opposite Real slab
+  // into the ghost slab (operator_ghost_manager.hpp:65-114, make_domain.hpp:213-255)
+  const int n[3] = {h->nx, h->ny, h->nz};
+  const int L[3] = {h->nx + 4, h->ny + 4, h->nz + 4};
+  const size_t Nall = (size_t)L[0] * L[1] * L[2];
+  for (int s = 0; s < 27; s++) {
+    int rel[3], t = s;
+    for (int d = 0; d < 3; d++) {
+      const int v = t % 3;
+      rel[d] = v - 1;
+      t = (t - v) / 3;
+    }
+    if (!rel[0] && !rel[1] && !rel[2]) continue;
+    bool exists = true;
+    for (int d = 0; d < 3; d++)
+      if (rel[d] && !h->d.periodic[d]) exists = false;
+    if (!exists) continue;
+    int lo[3], hi[3], sh[3];
+    for (int d = 0; d < 3; d++) {
+      if (rel[d] == -1) {
+        lo[d] = 0;
+        hi[d] = 1;
+        sh[d] = n[d];
+      } else if (rel[d] == 1) {
+        lo[d] = L[d] - 2;
+        hi[d] = L[d] - 1;
+        sh[d] = -n[d];
+      } else {
+        lo[d] = 2;
+        hi[d] = L[d] - 3;
+        sh[d] = 0;
+      }
+    }
+    for (int c = 0; c < ncomp; c++)
+      for (int z = lo[2]; z <= hi[2]; z++)
+        for (int y = lo[1]; y <= hi[1]; y++)
+          for (int x = lo[0]; x <= hi[0]; x++) {
+            const size_t dst = ((size_t)z * L[1] + y) * L[0] + x;
+            const size_t src = ((size_t)(z + sh[2]) * L[1] + (y + sh[1])) * L[0] + (x + sh[0]);
+            fref[(size_t)c * Nall + dst] = fref[(size_t)c * Nall + src];
+          }
+  }
+}
+
+int hlbm_get_populations_f64(hlbm_domain* h, double* f, int32_t layout) {
+  if (!h || !f) return fail(HLBM_ERR_ARG, "NULL argument");
+  int rc = materialize(h, 0);
+  if (rc) return rc;
+  rc = ensure_tmp(h);
+  if (rc) return rc;
+  std::vector<double> plane(layout == HLBM_LAYOUT_REF ? h->N : 0);
+  const size_t Nall = ref_count(h);
+  char* base = (char*)h->buf[h->cur];
+  for (int i = 0; i < Q; i++) {
+    double* dst = layout == HLBM_LAYOUT_REF ? plane.data() : f + (size_t)i * h->N;
+    void* srcp = base + (size_t)i * h->stride * h->esize;
+    if (h->d.storage == HLBM_F64) {
+      HIP_TRY(hipMemcpy(dst, srcp, (size_t)h->N * 8, hipMemcpyDeviceToHost));
+    } else {
+      HIP_TRY(launch_convert(h->d.storage, false, srcp, h->tmp, h->N, i, 0));
+      HIP_TRY(hipMemcpy(dst, h->tmp, (size_t)h->N * 8, hipMemcpyDeviceToHost));
+    }
+    if (layout == HLBM_LAYOUT_REF) {
+      size_t k = 0;
+      for (int z = 0; z < h->nz; z++)
+        for (int y = 0; y < h->ny; y++)
+          for (int x = 0; x < h->nx; x++, k++) f[(size_t)i * Nall + ref_index(h, x, y, z)] = plane[k];
+    }
+  }
+  if (layout == HLBM_LAYOUT_REF) fill_periodic_ghosts(h, f, Q);
+  return HLBM_OK;
+}
+
+int hlbm_ghost_update_ref(const hlbm_domain* h, double* f_ref) {
+  if (!h || !f_ref) return fail(HLBM_ERR_ARG, "NULL argument");
+  fill_periodic_ghosts(h, f_ref, Q);
+  return HLBM_OK;
+}
+
+int hlbm_step(hlbm_domain* h, int32_t nsteps, uint32_t flags, void* stream) {
+  if (!h || nsteps < 0) return fail(HLBM_ERR_ARG, "bad step arguments");
+  hipStream_t s = (hipStream_t)stream;
+  for (int k = 0; k < nsteps; k++) {
+    const bool wm = (flags & HLBM_STEP_WRITE_MACRO) && k == nsteps - 1;
+    const int mode = h->phase == PHASE_PRE ? MODE_COLLIDE : MODE_FUSED;
+    int rc = launch(h, mode, wm, s);
+    if (rc) return rc;
+    h->phase = PHASE_POST;
+    h->t++;
+  }
+  return HLBM_OK;
+}
+
+int hlbm_timestep(const hlbm_domain* h, int64_t* t) {
+  if (!h || !t) return fail(HLBM_ERR_ARG, "NULL argument");
+  *t = h->t;
+  return HLBM_OK;
+}
+
+int hlbm_synchronize(hlbm_domain* h) {
+  if (!h) return fail(HLBM_ERR_ARG, "NULL domain");
+  HIP_TRY(hipDeviceSynchronize());
+  return HLBM_OK;
+}
+
+static int copy_macro_out(hlbm_domain* h, double* rho, double* u, int32_t layout) {
+  const size_t N = h->N;
+  std::vector<double> r(N), v(3 * N);
+  HIP_TRY(hipMemcpy(r.data(), h->m0, N * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(v.data(), h->m1, 3 * N * 8, hipMemcpyDeviceToHost));
+  if (layout != HLBM_LAYOUT_REF) {
+    if (rho) std::memcpy(rho, r.data(), N * 8);
+    if (u) std::memcpy(u, v.data(), 3 * N * 8);
+    return HLBM_OK;
+  }
+  const size_t Nall = ref_count(h);
+  size_t k = 0;
+  for (int z = 0; z < h->nz; z++)
+    for (int y = 0; y < h->ny; y++)
+      for (int x = 0; x < h->nx; x++, k++) {
+        const size_t j = ref_index(h, x, y, z);
+        if (rho) rho[j] = r[k];
+        if (u)
+          for (int c = 0; c < 3; c++) u[c * Nall + j] = v[c * N + k];
+      }
+  return HLBM_OK;
+}
+
+int hlbm_macro(hlbm_domain* h, double* rho, double* u, int32_t layout) {
+  if (!h) return fail(HLBM_ERR_ARG, "NULL domain");
+  int rc = materialize(h, 0);
+  if (rc) return rc;
+  HIP_TRY(launch_macro(h->d.storage, h->buf[h->cur], h->stride, h->N, h->flags, h->m0, h->m1, 0));
+  HIP_TRY(hipDeviceSynchronize());
+  if (rho || u) return copy_macro_out(h, rho, u, layout);
+  return HLBM_OK;
+}
+
+int hlbm_get_macro(hlbm_domain* h, double* rho, double* u, int32_t layout) {
+  if (!h) return fail(HLBM_ERR_ARG, "NULL domain");
+  HIP_TRY(hipDeviceSynchronize());
+  return copy_macro_out(h, rho, u, layout);
+}
+
+int hlbm_reduce_stats(hlbm_domain* h, double out[7]) {
+  if (!h || !out) return fail(HLBM_ERR_ARG, "NULL argument");
+  HIP_TRY(launch_reduce(h->m0, h->m1, h->flags, h->N, h->red_part, h->red_out, 0));
+  HIP_TRY(hipMemcpy(out, h->red_out, 7 * 8, hipMemcpyDeviceToHost));
+  return HLBM_OK;
+}
+
+int hlbm_copy_bench(uint64_t bytes, int32_t reps, void* stream, double* ms_per_copy) {
+  if (bytes < 16 || reps < 1 || !ms_per_copy) return fail(HLBM_ERR_ARG, "bad copy benchmark arguments");
+  bytes &= ~(uint64_t)15;
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  int rc = HLBM_OK;
+  if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess) {
+    rc = fail(HLBM_ERR_NOMEM, "copy benchmark allocation failed");
+  } else if (hipMemset(a, 1, bytes) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+             hipEventCreate(&e1) != hipSuccess) {
+    rc = fail(HLBM_ERR_HIP, "copy benchmark setup failed");
+  } else {
+    hipError_t e = launch_copy(b, a, bytes, s);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, s);
+    for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_copy(b, a, bytes, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess)
+      rc = fail(HLBM_ERR_HIP, std::string("copy benchmark: ") + hipGetErrorString(e));
+    else
+      *ms_per_copy = (double)ms / reps;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
+}
+
+}  // extern "C"

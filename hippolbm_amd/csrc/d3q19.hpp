@@ -1,0 +1,56 @@
+// d3q19.hpp -- D3Q19 lattice constants and the per-node physics shared by the
+// HIP kernels (device) and the host library.
+//
+// Velocity ordering, weights and opposites follow hippoLBM's LBMScheme<19>
+// (src/grid/include/hippoLBM/grid/scheme.hpp:52-70): 0 rest; 1/2 +-x; 3/4 +-y;
+// 5/6 +-z; 7..18 the twelve diagonals as opposite pairs (odd i, i+1).
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define HLBM_HD __host__ __device__
+#else
+#define HLBM_HD
+#endif
+
+namespace hlbm {
+
+constexpr int Q = 19;
+
+struct Lattice {
+  static constexpr int cx[Q] = {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0};
+  static constexpr int cy[Q] = {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1};
+  static constexpr int cz[Q] = {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1};
+  // weight denominators: w_i = 1.0 / wden[i] (scheme.hpp:32)
+  static constexpr int wden[Q] = {3, 18, 18, 18, 18, 18, 18, 36, 36, 36, 36, 36, 36, 36, 36, 36, 36, 36, 36};
+  static constexpr int opp[Q] = {0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17};
+};
+
+HLBM_HD constexpr int CX(int i) { return Lattice::cx[i]; }
+HLBM_HD constexpr int CY(int i) { return Lattice::cy[i]; }
+HLBM_HD constexpr int CZ(int i) { return Lattice::cz[i]; }
+HLBM_HD constexpr int OPP(int i) { return Lattice::opp[i]; }
+HLBM_HD constexpr double WD(int i) { return 1.0 / Lattice::wden[i]; }
+HLBM_HD constexpr float WF(int i) { return (float)(1.0 / Lattice::wden[i]); }
+
+// Incoming directions on each domain face, in the order the reference's
+// post_bounce_back writes them (src/bcs/include/hippoLBM/bcs/bounce_back.hpp:115-140).
+// face = dim*2 + side (side 0 = low face, plane "_0"; 1 = high face, "_l").
+struct Faces {
+  static constexpr int in[6][5] = {{1, 9, 7, 11, 13}, {2, 10, 8, 12, 14}, {3, 7, 10, 15, 17},
+                                   {4, 8, 9, 16, 18}, {5, 14, 11, 18, 15}, {6, 13, 12, 17, 16}};
+};
+
+// Slot of direction i in the cavity coefficient table of a z face
+// (cavity_coeff, src/bcs/include/hippoLBM/bcs/cavity.hpp:39-40): the cavity on
+// the low z face adds to directions {5,14,11,18,15}, the high face to {6,13,12,17,16}.
+HLBM_HD constexpr int cavity_slot(int i) {
+  return i == 5 || i == 6 ? 0 : i == 14 || i == 13 ? 1 : i == 11 || i == 12 ? 2 : i == 18 || i == 17 ? 3 : 4;
+}
+
+// plane index (hlbm_plane) -> (dim, side)
+HLBM_HD constexpr int plane_dim(int p) { return p < 2 ? 2 : (p < 4 ? 1 : 0); }
+HLBM_HD constexpr int plane_side(int p) { return p & 1; }
+
+}  // namespace hlbm

@@ -1,0 +1,28 @@
+// launch.hpp -- host-side launchers for the step kernels (one translation unit
+// per floating-point contract mode).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "step_kernel.hpp"
+
+namespace hlbm {
+
+// Blocks of 256 threads covering every node of the program planes.
+inline unsigned boundary_blocks(const StepParams& p) {
+  uint64_t n = 0;
+  for (int pl = 0; pl < 6; pl++)
+    if ((p.prog_planes >> pl) & 1) {
+      const int dim = plane_dim(pl);
+      n += dim == 0 ? (uint64_t)p.ny * p.nz : (dim == 1 ? (uint64_t)p.nx * p.nz : (uint64_t)p.nx * p.ny);
+    }
+  return (unsigned)((n + 255) / 256);
+}
+
+// FP64 storage, FP64 compute, no contraction (step_f64.hip)
+hipError_t launch_step_f64(int coll, int mode, dim3 grid, dim3 block, const StepParams& p, hipStream_t s);
+// FP32 compute on shifted FP32/FP16/BF16 storage (step_lowp.hip); storage = hlbm_storage
+hipError_t launch_step_lowp(int storage, int coll, int mode, dim3 grid, dim3 block, const StepParams& p,
+                            hipStream_t s);
+
+}  // namespace hlbm

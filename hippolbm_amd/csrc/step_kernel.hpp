@@ -1,0 +1,722 @@
+// step_kernel.hpp -- the fused pull-scheme D3Q19 step for gfx950.
+//
+// One thread owns one lattice node.  Per step it
+//   1. gathers the 19 post-collision populations streaming into the node
+//      (pull: f_i(x) <- f*_i(x - c_i)), resolving in registers
+//        - periodic axes by index wrap (replaces streaming + update_ghost,
+//          src/collision/streaming.cu:107-111),
+//        - non-periodic domain faces by on-node bounce-back f_i(x) <- f*_opp(i)(x)
+//          (the implicit bounce-back of the reference's swap streaming,
+//          src/collision/include/hippoLBM/collision/streaming.hpp:57-66, which
+//          pre_bounce_back/post_bounce_back re-impose, bounce_back.hpp:92-156),
+//          plus the moving-wall term of cavity_z_0 / cavity_z_l (cavity.hpp:57-87),
+//        - obstacle cells by half-way bounce-back f_i(x) <- f*_opp(i)(x) when the
+//          upstream node is a wall (wall_bounce_back, bounce_back.hpp:163-185);
+//   2. runs the post-stream boundary program (post_bounce_back re-application,
+//      lid_driven_cavity, neumann) in the operator-list order of the deck;
+//   3. computes rho and u (macro_variables, macro_variables.hpp:48-79);
+//   4. collides (bgk.hpp:47-62 or mrt.hpp:42-193) and stores f*.
+//
+// The FP64 instantiation keeps the reference's evaluation order and is built
+// without floating-point contraction, so its populations are bit-identical to
+// the reference's CPU path.  The reduced-precision instantiations compute in
+// FP32 on shifted populations g_i = f_i - w_i stored as FP32/FP16/BF16.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <type_traits>
+
+#include "d3q19.hpp"
+
+namespace hlbm {
+
+enum StepMode { MODE_FUSED = 0, MODE_COLLIDE = 1, MODE_STREAM = 2 };
+enum ProgKind { PK_LID = 1, PK_NEUMANN = 2, PK_REBB = 3 };
+enum CollKind { COLL_BGK = 0, COLL_MRT = 1 };
+
+constexpr int MAX_PROG = 12;
+
+struct ProgEntry {
+  int32_t kind;
+  int32_t plane;  // hlbm_plane for LID/NEUMANN; unused for REBB
+  double U[3];
+};
+
+struct StepParams {
+  int32_t nx, ny, nz;
+  int32_t per[3];
+  uint32_t N;           // nodes
+  uint32_t pad0;
+  uint64_t stride;      // elements between two population planes
+  const void* fin;
+  void* fout;
+  const uint8_t* flags; // 0 fluid, 1 wall
+  double* m0;           // rho (written when write_macro)
+  double* m1;           // u, SoA 3*N
+  int32_t write_macro;
+  int32_t wall_bb;      // wall_bounce_back active
+  int32_t plane_bb;     // pre_bounce_back + post_bounce_back active
+  int32_t cav_on[2];    // cavity on the low / high z face
+  int32_t nprog;
+  int32_t prog_planes;  // OR of plane bits used by the program
+  int32_t mrt_custom;
+  double cav[2][5];
+  double tau;
+  double F[3];
+  double ef[Q];         // c_i . F, evaluated in the reference's order on the host
+  double mrt[10];       // s1, s2, s4, s9, s10, s13, s16, weps, wepsj, wxx (mrt_custom)
+  ProgEntry prog[MAX_PROG];
+};
+
+// ---------------------------------------------------------------------------
+// storage types
+// ---------------------------------------------------------------------------
+struct bf16s {
+  uint16_t v;
+};
+
+template <class S>
+struct StoreOps;
+
+template <>
+struct StoreOps<double> {
+  using C = double;
+  static constexpr bool shifted = false;
+  static __device__ __forceinline__ C ld(const double* p, size_t i) { return p[i]; }
+  static __device__ __forceinline__ void st(double* p, size_t i, C v) { p[i] = v; }
+};
+template <>
+struct StoreOps<float> {
+  using C = float;
+  static constexpr bool shifted = true;
+  static __device__ __forceinline__ C ld(const float* p, size_t i) { return p[i]; }
+  static __device__ __forceinline__ void st(float* p, size_t i, C v) { p[i] = v; }
+};
+template <>
+struct StoreOps<_Float16> {
+  using C = float;
+  static constexpr bool shifted = true;
+  static __device__ __forceinline__ C ld(const _Float16* p, size_t i) { return (float)p[i]; }
+  static __device__ __forceinline__ void st(_Float16* p, size_t i, C v) { p[i] = (_Float16)v; }
+};
+template <>
+struct StoreOps<bf16s> {
+  using C = float;
+  static constexpr bool shifted = true;
+  static __device__ __forceinline__ C ld(const bf16s* p, size_t i) {
+    return __uint_as_float(((uint32_t)p[i].v) << 16);
+  }
+  static __device__ __forceinline__ void st(bf16s* p, size_t i, C v) {
+    uint32_t u = __float_as_uint(v);
+    u += 0x7FFFu + ((u >> 16) & 1u);  // round to nearest even
+    bf16s r;
+    r.v = (uint16_t)(u >> 16);
+    p[i] = r;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// macroscopic variables
+// ---------------------------------------------------------------------------
+// FP64: rho and j accumulated in direction order (macro_variables.hpp:56-62).
+// Terms with a zero lattice coefficient are skipped and +-1 coefficients become
+// add/subtract; both are exact, so the sums are the reference's bit for bit.
+__device__ __forceinline__ void macro_f64(const double* f, double& rho, double& ux, double& uy, double& uz) {
+  rho = 0.0;
+  ux = 0.0;
+  uy = 0.0;
+  uz = 0.0;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    const double s = f[i];
+    if (CX(i) == 1) ux += s;
+    if (CX(i) == -1) ux -= s;
+    if (CY(i) == 1) uy += s;
+    if (CY(i) == -1) uy -= s;
+    if (CZ(i) == 1) uz += s;
+    if (CZ(i) == -1) uz -= s;
+    rho += s;
+  }
+  if (rho > 1.0e-14) {
+    ux /= rho;
+    uy /= rho;
+    uz /= rho;
+  }
+}
+
+// c_i . v in the reference's order (ex*vx + ey*vy) + ez*vz with zero terms dropped.
+template <int i, class C>
+__device__ __forceinline__ C cdot(C vx, C vy, C vz) {
+  C r = C(0);
+  bool first = true;
+  if (CX(i) != 0) {
+    r = CX(i) == 1 ? vx : -vx;
+    first = false;
+  }
+  if (CY(i) != 0) {
+    r = first ? (CY(i) == 1 ? vy : -vy) : (CY(i) == 1 ? r + vy : r - vy);
+    first = false;
+  }
+  if (CZ(i) != 0) {
+    r = first ? (CZ(i) == 1 ? vz : -vz) : (CZ(i) == 1 ? r + vz : r - vz);
+  }
+  return r;
+}
+
+template <int I = 0, int END = Q, class F>
+__device__ __forceinline__ void static_for(F&& fn) {
+  if constexpr (I < END) {
+    fn(std::integral_constant<int, I>{});
+    static_for<I + 1, END>(fn);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BGK (bgk.hpp:47-62), FP64 order-faithful
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void bgk_f64(double* f, double rho, double ux, double uy, double uz, const StepParams& p) {
+  const double u_squ = (ux * ux + uy * uy + uz * uz);
+  const double tau = p.tau;
+  static_for([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    const double w = WD(i);
+    const double eu = cdot<i>(ux, uy, uz);
+    const double feq = w * rho * (1. + 3. * eu + 4.5 * eu * eu - 1.5 * u_squ);
+    f[i] += (feq - f[i]) / tau + 3. * rho * w * p.ef[i];
+  });
+}
+
+// BGK on shifted populations g = f - w, FP32.
+__device__ __forceinline__ void bgk_shift(float* g, float dr, float ux, float uy, float uz, const StepParams& p,
+                                          float omega, const float* ef3) {
+  const float rho = 1.0f + dr;
+  const float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
+  static_for([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    const float w = WF(i);
+    const float eu = cdot<i>(ux, uy, uz);
+    const float geq = w * (dr + rho * (eu * (3.0f + 4.5f * eu) - usq15));
+    g[i] = g[i] + omega * (geq - g[i]) + rho * w * ef3[i];
+  });
+}
+
+// ---------------------------------------------------------------------------
+// MRT (mrt.hpp:42-193).  Moment rows and the inverse transform are the
+// d'Humieres D3Q19 basis; the FP64 path accumulates each row in increasing
+// direction order and each inverse row in moment order, as the reference does.
+// moment order: rho e eps jx qx jy qy jz qz pxx3 pixx3 pww piww pxy pyz pxz mx my mz
+// ---------------------------------------------------------------------------
+struct MrtBasis {
+  static constexpr int M[19][19] = {
+      {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
+      {-30, -11, -11, -11, -11, -11, -11, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8},
+      {12, -4, -4, -4, -4, -4, -4, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
+      {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0},
+      {0, -4, 4, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0},
+      {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1},
+      {0, 0, 0, -4, 4, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1},
+      {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1},
+      {0, 0, 0, 0, 0, -4, 4, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1},
+      {0, 2, 2, -1, -1, -1, -1, 1, 1, 1, 1, 1, 1, 1, 1, -2, -2, -2, -2},
+      {0, -4, -4, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, -2, -2, -2, -2},
+      {0, 0, 0, 1, 1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0},
+      {0, 0, 0, -2, -2, 2, 2, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1},
+      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, 1, -1, 1, -1, -1, 1, -1, 1, 0, 0, 0, 0},
+      {0, 0, 0, 0, 0, 0, 0, -1, 1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1},
+      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, 1, -1},
+  };
+  // inverse transform: f_i = sum_k sign*num/den * m_k; stored as signed numerator and denominator
+  static constexpr int NUM[19][19] = {
+      {1, -5, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
+      {1, -11, -1, 1, -1, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 0, 0, 0, 0},
+      {1, -11, -1, -1, 1, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 0, 0, 0, 0},
+      {1, -11, -1, 0, 0, 1, -1, 0, 0, -1, 1, 1, -1, 0, 0, 0, 0, 0, 0},
+      {1, -11, -1, 0, 0, -1, 1, 0, 0, -1, 1, 1, -1, 0, 0, 0, 0, 0, 0},
+      {1, -11, -1, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, 0, 0, 0, 0, 0, 0},
+      {1, -11, -1, 0, 0, 0, 0, -1, 1, -1, 1, -1, 1, 0, 0, 0, 0, 0, 0},
+      {1, 4, 1, 1, 1, 1, 1, 0, 0, 1, 1, 1, 1, 1, 0, 0, 1, -1, 0},
+      {1, 4, 1, -1, -1, -1, -1, 0, 0, 1, 1, 1, 1, 1, 0, 0, -1, 1, 0},
+      {1, 4, 1, 1, 1, -1, -1, 0, 0, 1, 1, 1, 1, -1, 0, 0, 1, 1, 0},
+      {1, 4, 1, -1, -1, 1, 1, 0, 0, 1, 1, 1, 1, -1, 0, 0, -1, -1, 0},
+      {1, 4, 1, 1, 1, 0, 0, 1, 1, 1, 1, -1, -1, 0, 0, 1, -1, 0, 1},
+      {1, 4, 1, -1, -1, 0, 0, -1, -1, 1, 1, -1, -1, 0, 0, 1, 1, 0, -1},
+      {1, 4, 1, 1, 1, 0, 0, -1, -1, 1, 1, -1, -1, 0, 0, -1, -1, 0, -1},
+      {1, 4, 1, -1, -1, 0, 0, 1, 1, 1, 1, -1, -1, 0, 0, -1, 1, 0, 1},
+      {1, 4, 1, 0, 0, 1, 1, 1, 1, -1, -1, 0, 0, 0, 1, 0, 0, 1, -1},
+      {1, 4, 1, 0, 0, -1, -1, -1, -1, -1, -1, 0, 0, 0, 1, 0, 0, -1, 1},
+      {1, 4, 1, 0, 0, 1, 1, -1, -1, -1, -1, 0, 0, 0, -1, 0, 0, 1, 1},
+      {1, 4, 1, 0, 0, -1, -1, 1, 1, -1, -1, 0, 0, 0, -1, 0, 0, -1, -1},
+  };
+  static constexpr int DEN[19][19] = {
+      {19, 399, 21, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
+      {19, 2394, 63, 10, 10, 1, 1, 1, 1, 18, 18, 1, 1, 1, 1, 1, 1, 1, 1},
+      {19, 2394, 63, 10, 10, 1, 1, 1, 1, 18, 18, 1, 1, 1, 1, 1, 1, 1, 1},
+      {19, 2394, 63, 1, 1, 10, 10, 1, 1, 36, 36, 12, 12, 1, 1, 1, 1, 1, 1},
+      {19, 2394, 63, 1, 1, 10, 10, 1, 1, 36, 36, 12, 12, 1, 1, 1, 1, 1, 1},
+      {19, 2394, 63, 1, 1, 1, 1, 10, 10, 36, 36, 12, 12, 1, 1, 1, 1, 1, 1},
+      {19, 2394, 63, 1, 1, 1, 1, 10, 10, 36, 36, 12, 12, 1, 1, 1, 1, 1, 1},
+      {19, 1197, 252, 10, 40, 10, 40, 1, 1, 36, 72, 12, 24, 4, 1, 1, 8, 8, 1},
+      {19, 1197, 252, 10, 40, 10, 40, 1, 1, 36, 72, 12, 24, 4, 1, 1, 8, 8, 1},
+      {19, 1197, 252, 10, 40, 10, 40, 1, 1, 36, 72, 12, 24, 4, 1, 1, 8, 8, 1},
+      {19, 1197, 252, 10, 40, 10, 40, 1, 1, 36, 72, 12, 24, 4, 1, 1, 8, 8, 1},
+      {19, 1197, 252, 10, 40, 1, 1, 10, 40, 36, 72, 12, 24, 1, 1, 4, 8, 1, 8},
+      {19, 1197, 252, 10, 40, 1, 1, 10, 40, 36, 72, 12, 24, 1, 1, 4, 8, 1, 8},
+      {19, 1197, 252, 10, 40, 1, 1, 10, 40, 36, 72, 12, 24, 1, 1, 4, 8, 1, 8},
+      {19, 1197, 252, 10, 40, 1, 1, 10, 40, 36, 72, 12, 24, 1, 1, 4, 8, 1, 8},
+      {19, 1197, 252, 1, 1, 10, 40, 10, 40, 18, 36, 1, 1, 1, 4, 1, 1, 8, 8},
+      {19, 1197, 252, 1, 1, 10, 40, 10, 40, 18, 36, 1, 1, 1, 4, 1, 1, 8, 8},
+      {19, 1197, 252, 1, 1, 10, 40, 10, 40, 18, 36, 1, 1, 1, 4, 1, 1, 8, 8},
+      {19, 1197, 252, 1, 1, 10, 40, 10, 40, 18, 36, 1, 1, 1, 4, 1, 1, 8, 8},
+  };
+};
+
+// Row k of M applied to f, accumulated left to right over the non-zero entries.
+template <int k, class C>
+__device__ __forceinline__ C moment(const C* f) {
+  C acc = C(0);
+  bool first = true;
+  static_for([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    constexpr int c = MrtBasis::M[k][i];
+    if constexpr (c != 0) {
+      C term;
+      if constexpr (c == 1 || c == -1)
+        term = f[i];
+      else
+        term = C(c < 0 ? -c : c) * f[i];
+      if (first) {
+        acc = c < 0 ? -term : term;
+        first = false;
+      } else {
+        acc = c < 0 ? acc - term : acc + term;
+      }
+    }
+  });
+  return acc;
+}
+
+// Inverse row i applied to the post-collision moments mo[19].
+template <int i, class C>
+__device__ __forceinline__ C inverse_row(const C* mo) {
+  C acc = C(0);
+  bool first = true;
+#pragma unroll
+  for (int k = 0; k < 19; k++) {
+    const int num = MrtBasis::NUM[i][k];
+    if (num != 0) {
+      const C coef = C((double)(num < 0 ? -num : num) / (double)MrtBasis::DEN[i][k]);
+      const C term = coef * mo[k];
+      if (first) {
+        acc = num < 0 ? -term : term;
+        first = false;
+      } else {
+        acc = num < 0 ? acc - term : acc + term;
+      }
+    }
+  }
+  return acc;
+}
+
+struct MrtRates {
+  double s1, s2, s4, s9, s10, s13, s16, weps, wepsj, wxx;
+};
+
+__device__ __forceinline__ MrtRates mrt_rates(const StepParams& p) {
+  MrtRates r;
+  if (p.mrt_custom) {
+    r.s1 = p.mrt[0];
+    r.s2 = p.mrt[1];
+    r.s4 = p.mrt[2];
+    r.s9 = p.mrt[3];
+    r.s10 = p.mrt[4];
+    r.s13 = p.mrt[5];
+    r.s16 = p.mrt[6];
+    r.weps = p.mrt[7];
+    r.wepsj = p.mrt[8];
+    r.wxx = p.mrt[9];
+  } else {  // mrt.hpp:43-47
+    r.s9 = 1 / p.tau;
+    r.s13 = r.s9;
+    r.s1 = 1.19;
+    r.s2 = 1.4;
+    r.s4 = 1.2;
+    r.s10 = r.s2;
+    r.s16 = 1.98;
+    r.weps = 0;
+    r.wepsj = -475. / 63.;
+    r.wxx = 0;
+  }
+  return r;
+}
+
+// Relaxation of the moments (mrt.hpp:96-126).  For the shifted form the
+// caller passes the moments of g = f - w; the rest state w has the moments
+// rho = 1, e = -11, eps = 3 (all others 0), which enter through rest_e/rest_eps
+// and rest_rho (all zero for the unshifted FP64 path).
+template <class C>
+__device__ __forceinline__ void mrt_relax(C* m, const MrtRates& r, C rest_rho, C rest_e, C rest_eps) {
+  const C rho = m[0] + rest_rho;
+  const C e = m[1], eps = m[2], jx = m[3], qx = m[4], jy = m[5], qy = m[6], jz = m[7], qz = m[8];
+  const C pxx3 = m[9], pixx3 = m[10], pww = m[11], piww = m[12], pxy = m[13], pyz = m[14], pxz = m[15];
+  const C mx = m[16], my = m[17], mz = m[18];
+  const C s1 = C(r.s1), s2 = C(r.s2), s4 = C(r.s4), s9 = C(r.s9), s10 = C(r.s10), s13 = C(r.s13), s16 = C(r.s16);
+  const C jx2 = jx * jx, jy2 = jy * jy, jz2 = jz * jz;
+  const C jj = (jx2 + jy2 + jz2);
+  // rho0 = 1: 19/rho0, wepsj/rho0 and 1/rho0 are exact
+  const C e_eq = C(-11) * rho + C(19) * jj - rest_e;
+  m[1] = e - s1 * (e - e_eq);
+  const C eps_eq = C(r.weps) * rho + C(r.wepsj) * jj - rest_eps;
+  m[2] = eps - s2 * (eps - eps_eq);
+  m[4] = qx - s4 * (qx + C(2. / 3.) * jx);
+  m[6] = qy - s4 * (qy + C(2. / 3.) * jy);
+  m[8] = qz - s4 * (qz + C(2. / 3.) * jz);
+  const C pxx3_eq = (C(2) * jx2 - (jy2 + jz2));
+  m[9] = pxx3 - s9 * (pxx3 - pxx3_eq);
+  m[10] = pixx3 - s10 * (pixx3 - C(r.wxx) * pxx3_eq);
+  const C pww_eq = (jy2 - jz2);
+  m[11] = pww - s9 * (pww - pww_eq);
+  m[12] = piww - s10 * (piww - C(r.wxx) * pww_eq);
+  const C pxy_eq = jx * jy;
+  m[13] = pxy - s13 * (pxy - pxy_eq);
+  const C pyz_eq = jy * jz;
+  m[14] = pyz - s13 * (pyz - pyz_eq);
+  const C pxz_eq = jx * jz;
+  m[15] = pxz - s13 * (pxz - pxz_eq);
+  m[16] = mx - s16 * mx;
+  m[17] = my - s16 * my;
+  m[18] = mz - s16 * mz;
+}
+
+template <class C>
+__device__ __forceinline__ void mrt_collide(C* f, C rho_force, const StepParams& p, bool shifted) {
+  C m[19];
+  static_for([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    m[k] = moment<k>(f);
+  });
+  const MrtRates r = mrt_rates(p);
+  if (shifted)
+    mrt_relax<C>(m, r, C(1), C(-11), C(3));
+  else
+    mrt_relax<C>(m, r, C(0), C(0), C(0));
+  static_for([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    f[i] = inverse_row<i>(m);
+  });
+  // mrt.hpp:187-191, force term with rho from macro_variables
+  static_for([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    f[i] += C(3.) * rho_force * C(WD(i)) * C(p.ef[i]);
+  });
+}
+
+// ---------------------------------------------------------------------------
+// post-stream boundary operators on one node
+// ---------------------------------------------------------------------------
+// lid_driven_cavity (lid_driven_cavity.hpp:52-62): f_i = feq(rho_local, U)
+template <class C, bool SHIFT>
+__device__ __forceinline__ void lid_node(C* f, const double* U) {
+  C rho = C(0);
+#pragma unroll
+  for (int i = 0; i < Q; i++) rho += f[i];
+  const C Ux = C(U[0]), Uy = C(U[1]), Uz = C(U[2]);
+  const C u_squ = Ux * Ux + Uy * Uy + Uz * Uz;
+  if constexpr (!SHIFT) {
+    static_for([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const C eu = C(CX(i)) * Ux + C(CY(i)) * Uy + C(CZ(i)) * Uz;
+      f[i] = C(WD(i)) * rho * (C(1.) + C(3.) * eu + C(4.5) * eu * eu - C(1.5) * u_squ);
+    });
+  } else {
+    const C rhot = C(1) + rho;  // rho holds sum g = rho - 1
+    static_for([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const C eu = C(CX(i)) * Ux + C(CY(i)) * Uy + C(CZ(i)) * Uz;
+      f[i] = C(WD(i)) * (rho + rhot * (C(3.) * eu + C(4.5) * eu * eu - C(1.5) * u_squ));
+    });
+  }
+}
+
+// neumann (Zou/He velocity boundary, neumann.hpp:38-258), one face.
+// SHIFT adds the rest-state weight sum (=1) to the density numerator.
+template <class C, bool SHIFT>
+__device__ __forceinline__ void neumann_node(C* f, int plane, const double* U) {
+  const C ux = C(U[0]), uy = C(U[1]), uz = C(U[2]);
+  const C base = SHIFT ? C(1) : C(0);
+  const C h = C(1. / 2.), t = C(1. / 3.), s = C(1. / 6.);
+  switch (plane) {
+    case 5: {  // plan_yz_l
+      const C rho = ((f[3] + f[4] + f[5] + f[6] + f[15] + f[17] + f[18] + f[16] + f[0] +
+                      C(2.) * (f[1] + f[7] + f[9] + f[11] + f[13])) + base) / (C(1.) + ux);
+      const C nyx = h * (f[3] + f[15] + f[17] - (f[4] + f[18] + f[16])) - t * rho * uy;
+      const C nzx = h * (f[5] + f[18] + f[15] - (f[6] + f[17] + f[16])) - t * rho * uz;
+      f[2] = f[1] - t * rho * ux;
+      f[10] = f[9] + s * rho * (-ux + uy) - nyx;
+      f[8] = f[7] + s * rho * (-ux - uy) + nyx;
+      f[12] = f[11] + s * rho * (-ux - uz) + nzx;
+      f[14] = f[13] + s * rho * (-ux + uz) - nzx;
+      break;
+    }
+    case 4: {  // plan_yz_0
+      const C rho = ((f[3] + f[4] + f[5] + f[6] + f[15] + f[17] + f[18] + f[16] + f[0] +
+                      C(2.) * (f[2] + f[10] + f[8] + f[14] + f[12])) + base) / (C(1.) - ux);
+      const C nyx = h * (f[3] + f[15] + f[17] - (f[4] + f[18] + f[16])) - t * rho * uy;
+      const C nzx = h * (f[5] + f[18] + f[15] - (f[6] + f[17] + f[16])) - t * rho * uz;
+      f[1] = f[2] + t * rho * ux;
+      f[9] = f[10] + s * rho * (ux - uy) + nyx;
+      f[7] = f[8] + s * rho * (ux + uy) - nyx;
+      f[11] = f[12] + s * rho * (ux + uz) - nzx;
+      f[13] = f[14] + s * rho * (ux - uz) + nzx;
+      break;
+    }
+    case 3: {  // plan_xz_l
+      const C rho = ((f[1] + f[2] + f[5] + f[6] + f[11] + f[13] + f[14] + f[12] + f[0] +
+                      C(2.) * (f[3] + f[7] + f[10] + f[15] + f[17])) + base) / (C(1.) + uy);
+      const C nxy = h * (f[1] + f[11] + f[13] - (f[2] + f[14] + f[12])) - t * rho * ux;
+      const C nzy = h * (f[5] + f[11] + f[14] - (f[6] + f[13] + f[12])) - t * rho * uz;
+      f[4] = f[3] - t * rho * uy;
+      f[8] = f[7] + s * rho * (-uy - ux) + nxy;
+      f[9] = f[10] + s * rho * (-uy + ux) - nxy;
+      f[16] = f[15] + s * rho * (-uy - uz) + nzy;
+      f[18] = f[17] + s * rho * (-uy + uz) - nzy;
+      break;
+    }
+    case 2: {  // plan_xz_0
+      const C rho = ((f[1] + f[2] + f[5] + f[6] + f[11] + f[13] + f[14] + f[12] + f[0] +
+                      C(2.) * (f[4] + f[9] + f[8] + f[18] + f[16])) + base) / (C(1.) - uy);
+      const C nxy = h * (f[1] + f[11] + f[13] - (f[2] + f[14] + f[12])) - t * rho * ux;
+      const C nzy = h * (f[5] + f[11] + f[14] - (f[6] + f[13] + f[12])) - t * rho * uz;
+      f[3] = f[4] + t * rho * uy;
+      f[7] = f[8] + s * rho * (uy + ux) - nxy;
+      f[10] = f[9] + s * rho * (uy - ux) + nxy;
+      f[15] = f[16] + s * rho * (uy + uz) - nzy;
+      f[17] = f[18] + s * rho * (uy - uz) + nzy;
+      break;
+    }
+    case 1: {  // plan_xy_l
+      const C rho = ((f[0] + f[1] + f[2] + f[3] + f[4] + f[7] + f[9] + f[10] + f[8] +
+                      C(2.) * (f[5] + f[11] + f[14] + f[15] + f[18])) + base) / (C(1.) + uz);
+      const C nxz = h * (f[1] + f[7] + f[9] - (f[2] + f[10] + f[8])) - t * rho * ux;
+      const C nyz = h * (f[3] + f[7] + f[10] - (f[4] + f[9] + f[8])) - t * rho * uy;
+      f[6] = f[5] - t * rho * uz;
+      f[13] = f[14] + s * rho * (-uz + ux) - nxz;
+      f[12] = f[11] + s * rho * (-uz - ux) + nxz;
+      f[17] = f[18] + s * rho * (-uz + uy) - nyz;
+      f[16] = f[15] + s * rho * (-uz - uy) + nyz;
+      break;
+    }
+    default: {  // 0: plan_xy_0
+      const C rho = ((f[0] + f[1] + f[2] + f[3] + f[4] + f[7] + f[9] + f[10] + f[8] +
+                      C(2.) * (f[6] + f[13] + f[12] + f[17] + f[16])) + base) / (C(1.) - uz);
+      const C nxz = h * (f[1] + f[7] + f[9] - (f[2] + f[10] + f[8])) - t * rho * ux;
+      const C nyz = h * (f[3] + f[7] + f[10] - (f[4] + f[9] + f[8])) - t * rho * uy;
+      f[5] = f[6] + t * rho * uz;
+      f[11] = f[12] + s * rho * (uz + ux) - nxz;
+      f[14] = f[13] + s * rho * (uz - ux) + nxz;
+      f[15] = f[16] + s * rho * (uz + uy) - nyz;
+      f[18] = f[17] + s * rho * (uz - uy) + nyz;
+      break;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// the kernel
+// ---------------------------------------------------------------------------
+// Plane bits of a node: bit p set when the node lies on plane p (hlbm_plane order).
+__device__ __forceinline__ int node_planes(int x, int y, int z, int nx, int ny, int nz) {
+  return (z == 0 ? 1 : 0) | (z == nz - 1 ? 2 : 0) | (y == 0 ? 4 : 0) | (y == ny - 1 ? 8 : 0) | (x == 0 ? 16 : 0) |
+         (x == nx - 1 ? 32 : 0);
+}
+
+// Full per-node pipeline.  PROG selects whether the post-stream boundary
+// program is compiled in: the bulk kernel never runs it (its nodes are not on
+// a program plane), the boundary kernel always may.
+template <class S, int COLL, int MODE, bool PROG>
+__device__ __forceinline__ void process_node(const StepParams& p, int x, int y, int z) {
+  using O = StoreOps<S>;
+  using C = typename O::C;
+  constexpr bool SHIFT = O::shifted;
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const uint32_t idx = ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx + (uint32_t)x;
+  const S* __restrict__ fin = (const S*)p.fin;
+  S* __restrict__ fout = (S*)p.fout;
+  const size_t st = p.stride;
+  const bool wall = p.flags[idx] != 0;
+
+  C f[Q];
+  if constexpr (MODE == MODE_COLLIDE) {
+#pragma unroll
+    for (int i = 0; i < Q; i++) f[i] = O::ld(fin, (size_t)i * st + idx);
+  } else {
+    // upstream coordinates; -1 marks a non-periodic face (bounce-back)
+    const int xm = x > 0 ? x - 1 : (p.per[0] ? nx - 1 : -1);
+    const int xp = x < nx - 1 ? x + 1 : (p.per[0] ? 0 : -1);
+    const int ym = y > 0 ? y - 1 : (p.per[1] ? ny - 1 : -1);
+    const int yp = y < ny - 1 ? y + 1 : (p.per[1] ? 0 : -1);
+    const int zm = z > 0 ? z - 1 : (p.per[2] ? nz - 1 : -1);
+    const int zp = z < nz - 1 ? z + 1 : (p.per[2] ? 0 : -1);
+    const bool wbb = p.wall_bb && !wall;
+    static_for([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const int sx = CX(i) == 0 ? x : (CX(i) == 1 ? xm : xp);
+      const int sy = CY(i) == 0 ? y : (CY(i) == 1 ? ym : yp);
+      const int sz = CZ(i) == 0 ? z : (CZ(i) == 1 ? zm : zp);
+      if (sx < 0 || sy < 0 || sz < 0) {
+        C v = O::ld(fin, (size_t)OPP(i) * st + idx);
+        if constexpr (CZ(i) != 0) {
+          constexpr int side = CZ(i) == 1 ? 0 : 1;
+          if (sz < 0 && p.cav_on[side]) v += C(p.cav[side][cavity_slot(i)]);
+        }
+        f[i] = v;
+      } else {
+        const uint32_t sidx = ((uint32_t)sz * (uint32_t)ny + (uint32_t)sy) * (uint32_t)nx + (uint32_t)sx;
+        if (i != 0 && wbb && p.flags[sidx] != 0)
+          f[i] = O::ld(fin, (size_t)OPP(i) * st + idx);
+        else
+          f[i] = O::ld(fin, (size_t)i * st + sidx);
+      }
+    });
+
+    // post-stream program
+    if constexpr (PROG) {
+      const int bmask = node_planes(x, y, z, nx, ny, nz);
+      {
+        for (int k = 0; k < p.nprog; k++) {
+          const ProgEntry& e = p.prog[k];
+          if (e.kind == PK_REBB) {
+            // post_bounce_back: faces in x-left, x-right, y-left, y-right, z-left, z-right order
+            static_for<0, 6>([&](auto fc) {
+              constexpr int face = decltype(fc)::value;
+              constexpr int dim = face >> 1, side = face & 1;
+              constexpr int pl = dim == 2 ? side : (dim == 1 ? 2 + side : 4 + side);
+              if (!p.per[dim] && ((bmask >> pl) & 1)) {
+                static_for<0, 5>([&](auto jc) {
+                  constexpr int j = decltype(jc)::value;
+                  constexpr int qi = Faces::in[face][j];
+                  C v = O::ld(fin, (size_t)OPP(qi) * st + idx);
+                  if (dim == 2 && p.cav_on[side]) v += C(p.cav[side][j]);
+                  f[qi] = v;
+                });
+              }
+            });
+          } else if ((bmask >> e.plane) & 1) {
+            if (!wall) {
+              if (e.kind == PK_LID)
+                lid_node<C, SHIFT>(f, e.U);
+              else
+                neumann_node<C, SHIFT>(f, e.plane, e.U);
+            }
+          }
+        }
+      }
+    }
+  }
+
+  if constexpr (MODE != MODE_STREAM) {
+    if constexpr (!SHIFT) {
+      double rho, ux, uy, uz;
+      macro_f64((const double*)f, rho, ux, uy, uz);
+      if (p.write_macro) {
+        if (!wall) {
+          p.m0[idx] = rho;
+          p.m1[idx] = ux;
+          p.m1[(size_t)p.N + idx] = uy;
+          p.m1[2 * (size_t)p.N + idx] = uz;
+        } else {
+          p.m1[idx] = 0.0;
+          p.m1[(size_t)p.N + idx] = 0.0;
+          p.m1[2 * (size_t)p.N + idx] = 0.0;
+        }
+      }
+      if (!wall) {
+        if constexpr (COLL == COLL_BGK)
+          bgk_f64((double*)f, rho, ux, uy, uz, p);
+        else
+          mrt_collide<double>((double*)f, rho, p, false);
+      }
+    } else {
+      float dr = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+#pragma unroll
+      for (int i = 0; i < Q; i++) {
+        dr += f[i];
+        jx += float(CX(i)) * f[i];
+        jy += float(CY(i)) * f[i];
+        jz += float(CZ(i)) * f[i];
+      }
+      const float rho = 1.0f + dr;
+      const float inv = 1.0f / rho;
+      const float ux = jx * inv, uy = jy * inv, uz = jz * inv;
+      if (p.write_macro) {
+        if (!wall) {
+          p.m0[idx] = (double)rho;
+          p.m1[idx] = (double)ux;
+          p.m1[(size_t)p.N + idx] = (double)uy;
+          p.m1[2 * (size_t)p.N + idx] = (double)uz;
+        } else {
+          p.m1[idx] = 0.0;
+          p.m1[(size_t)p.N + idx] = 0.0;
+          p.m1[2 * (size_t)p.N + idx] = 0.0;
+        }
+      }
+      if (!wall) {
+        if constexpr (COLL == COLL_BGK) {
+          float ef3[Q];
+#pragma unroll
+          for (int i = 0; i < Q; i++) ef3[i] = 3.0f * (float)p.ef[i];
+          bgk_shift((float*)f, dr, ux, uy, uz, p, (float)(1.0 / p.tau), ef3);
+        } else {
+          mrt_collide<float>((float*)f, rho, p, true);
+        }
+      }
+    }
+  }
+
+#pragma unroll
+  for (int i = 0; i < Q; i++) O::st(fout, (size_t)i * st + idx, f[i]);
+}
+
+
+// Bulk kernel: every node that is not on a plane of the boundary program.
+template <class S, int COLL, int MODE>
+__global__ void __launch_bounds__(256) step_kernel(const StepParams p) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= p.nx || y >= p.ny) return;
+  if (MODE != MODE_COLLIDE && (node_planes(x, y, z, p.nx, p.ny, p.nz) & p.prog_planes)) return;
+  process_node<S, COLL, MODE, false>(p, x, y, z);
+}
+
+// Boundary kernel: the nodes of the program planes, each once (a node on
+// several planes is handled by the lowest-numbered plane that holds it).
+template <class S, int COLL, int MODE>
+__global__ void __launch_bounds__(256) boundary_kernel(const StepParams p) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  for (int pl = 0; pl < 6; pl++) {
+    if (!((p.prog_planes >> pl) & 1)) continue;
+    const int dim = plane_dim(pl);
+    const uint32_t na = dim == 0 ? (uint32_t)ny : (uint32_t)nx;  // fastest in-plane axis
+    const uint32_t nb = dim == 2 ? (uint32_t)ny : (uint32_t)nz;
+    const uint32_t cnt = na * nb;
+    if (t < cnt) {
+      const int a = (int)(t % na), b = (int)(t / na);
+      const int c = plane_side(pl) ? (dim == 0 ? nx - 1 : (dim == 1 ? ny - 1 : nz - 1)) : 0;
+      const int x = dim == 0 ? c : a;
+      const int y = dim == 0 ? a : (dim == 1 ? c : b);
+      const int z = dim == 2 ? c : b;
+      const int lower = node_planes(x, y, z, nx, ny, nz) & p.prog_planes & ((1 << pl) - 1);
+      if (!lower) process_node<S, COLL, MODE, true>(p, x, y, z);
+      return;
+    }
+    t -= cnt;
+  }
+}
+
+}  // namespace hlbm

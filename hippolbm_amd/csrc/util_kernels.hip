@@ -1,0 +1,312 @@
+// util_kernels.hip -- initialisation, conversion, macro and reduction kernels.
+// Compiled with -ffp-contract=off (obstacle rasterization and the FP64 macro
+// must round exactly as hippoLBM's CPU build does).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "step_kernel.hpp"
+#include "util_kernels.hpp"
+
+namespace hlbm {
+
+// ---------------------------------------------------------------------------
+// update_obstacles (src/obstacle/update_obstacles.cu:32-71): for each obstacle
+// in id order, nodes inside its truncated bounding box (host-computed, already
+// clipped to the grid) are tested with Ball::solid (|p-c|^2 <= r^2) or
+// Wall::solid (strict AABB), obstacle.hpp:13-17,61-64,97.  Position of a node
+// is (local + offset) * dx = node * dx (grid.hpp:355-361).
+// ---------------------------------------------------------------------------
+__global__ void k_rasterize(uint8_t* flags, int nx, int ny, int nz, double dx, const ObstacleDev* obs, int nobs) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= nx || y >= ny) return;
+  const size_t idx = ((size_t)z * ny + y) * nx + x;
+  // local index i = node + 2, offset = -2: (double(i) + double(-2)) * dx
+  const double px = (double(x + 2) + double(-2)) * dx;
+  const double py = (double(y + 2) + double(-2)) * dx;
+  const double pz = (double(z + 2) + double(-2)) * dx;
+  bool solid = false;
+  for (int k = 0; k < nobs; k++) {
+    const ObstacleDev o = obs[k];
+    if (x < o.lo[0] || x > o.hi[0] || y < o.lo[1] || y > o.hi[1] || z < o.lo[2] || z > o.hi[2]) continue;
+    if (o.kind == 0) {
+      const double rx = px - o.p0[0], ry = py - o.p0[1], rz = pz - o.p0[2];
+      if ((rx * rx + ry * ry + rz * rz) <= o.r2) solid = true;
+    } else {
+      if (o.p0[0] < px && px < o.p1[0] && o.p0[1] < py && py < o.p1[1] && o.p0[2] < pz && pz < o.p1[2])
+        solid = true;
+    }
+  }
+  if (solid) flags[idx] = 1;
+}
+
+// set_distribution (set_distribution.hpp:38-41): f_i = value * w_i on a node box.
+template <class S>
+__global__ void k_set_dist(void* fv, uint64_t stride, int nx, int ny, int lo0, int lo1, int lo2, int hi0, int hi1,
+                           int hi2, double value) {
+  const int x = lo0 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = lo1 + blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = lo2 + blockIdx.z;
+  if (x > hi0 || y > hi1 || z > hi2) return;
+  const size_t idx = ((size_t)z * ny + y) * nx + x;
+  S* f = (S*)fv;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    const double v = value * WD(i);
+    if constexpr (StoreOps<S>::shifted)
+      StoreOps<S>::st(f, (size_t)i * stride + idx, (float)(v - WD(i)));
+    else
+      StoreOps<S>::st(f, (size_t)i * stride + idx, v);
+  }
+}
+
+// FP64 plane -> storage plane (adds the shift)
+template <class S>
+__global__ void k_from_f64(void* fv, const double* src, uint32_t n, int q) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  S* f = (S*)fv;
+  if constexpr (StoreOps<S>::shifted)
+    StoreOps<S>::st(f, i, (float)(src[i] - WD(q)));
+  else
+    StoreOps<S>::st(f, i, src[i]);
+}
+
+template <class S>
+__global__ void k_to_f64(double* dst, const void* fv, uint32_t n, int q) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const S* f = (const S*)fv;
+  if constexpr (StoreOps<S>::shifted)
+    dst[i] = (double)StoreOps<S>::ld(f, i) + WD(q);
+  else
+    dst[i] = StoreOps<S>::ld(f, i);
+}
+
+// macro_variables on the current (pre-collision) state, macro_variables.hpp:48-79.
+template <class S>
+__global__ void k_macro(const void* fv, uint64_t stride, uint32_t N, const uint8_t* flags, double* m0, double* m1) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  const S* f = (const S*)fv;
+  if (flags[idx]) {
+    m1[idx] = 0.0;
+    m1[(size_t)N + idx] = 0.0;
+    m1[2 * (size_t)N + idx] = 0.0;
+    return;
+  }
+  if constexpr (!StoreOps<S>::shifted) {
+    double fl[Q];
+#pragma unroll
+    for (int i = 0; i < Q; i++) fl[i] = StoreOps<S>::ld(f, (size_t)i * stride + idx);
+    double rho, ux, uy, uz;
+    macro_f64(fl, rho, ux, uy, uz);
+    m0[idx] = rho;
+    m1[idx] = ux;
+    m1[(size_t)N + idx] = uy;
+    m1[2 * (size_t)N + idx] = uz;
+  } else {
+    float dr = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+#pragma unroll
+    for (int i = 0; i < Q; i++) {
+      const float g = StoreOps<S>::ld(f, (size_t)i * stride + idx);
+      dr += g;
+      jx += float(CX(i)) * g;
+      jy += float(CY(i)) * g;
+      jz += float(CZ(i)) * g;
+    }
+    const float rho = 1.0f + dr;
+    m0[idx] = rho;
+    m1[idx] = jx / rho;
+    m1[(size_t)N + idx] = jy / rho;
+    m1[2 * (size_t)N + idx] = jz / rho;
+  }
+}
+
+// simulation_state (simulation_state.hpp:47-79): per-block partial sums in
+// FP64, then one block folds the partials in a fixed order (deterministic).
+constexpr int RED_BLOCK = 256;
+
+__global__ void __launch_bounds__(RED_BLOCK)
+    k_reduce_partial(const double* m0, const double* m1, const uint8_t* flags, uint32_t N, double* part) {
+  __shared__ double s_sum[RED_BLOCK], s_min[RED_BLOCK], s_max[RED_BLOCK], s_jx[RED_BLOCK], s_jy[RED_BLOCK],
+      s_jz[RED_BLOCK], s_cnt[RED_BLOCK];
+  double sum = 0.0, vmin = 1.7976931348623157e308, vmax = -1.7976931348623157e308, jx = 0.0, jy = 0.0, jz = 0.0,
+         cnt = 0.0;
+  const uint32_t stride = gridDim.x * RED_BLOCK;
+  for (uint32_t i = blockIdx.x * RED_BLOCK + threadIdx.x; i < N; i += stride) {
+    const double r = m0[i];
+    const double ux = m1[i], uy = m1[(size_t)N + i], uz = m1[2 * (size_t)N + i];
+    const double v = sqrt(ux * ux + uy * uy + uz * uz);
+    sum += r;
+    vmin = fmin(vmin, v);
+    vmax = fmax(vmax, v);
+    jx += r * ux;
+    jy += r * uy;
+    jz += r * uz;
+    cnt += flags[i] ? 0.0 : 1.0;
+  }
+  const int t = threadIdx.x;
+  s_sum[t] = sum;
+  s_min[t] = vmin;
+  s_max[t] = vmax;
+  s_jx[t] = jx;
+  s_jy[t] = jy;
+  s_jz[t] = jz;
+  s_cnt[t] = cnt;
+  __syncthreads();
+  for (int w = RED_BLOCK / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      s_sum[t] += s_sum[t + w];
+      s_min[t] = fmin(s_min[t], s_min[t + w]);
+      s_max[t] = fmax(s_max[t], s_max[t + w]);
+      s_jx[t] += s_jx[t + w];
+      s_jy[t] += s_jy[t + w];
+      s_jz[t] += s_jz[t + w];
+      s_cnt[t] += s_cnt[t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    double* o = part + 7 * (size_t)blockIdx.x;
+    o[0] = s_sum[0];
+    o[1] = s_min[0];
+    o[2] = s_max[0];
+    o[3] = s_jx[0];
+    o[4] = s_jy[0];
+    o[5] = s_jz[0];
+    o[6] = s_cnt[0];
+  }
+}
+
+__global__ void k_reduce_final(const double* part, int nparts, double* out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double r[7] = {0.0, 1.7976931348623157e308, -1.7976931348623157e308, 0.0, 0.0, 0.0, 0.0};
+  for (int b = 0; b < nparts; b++) {
+    const double* o = part + 7 * (size_t)b;
+    r[0] += o[0];
+    r[1] = fmin(r[1], o[1]);
+    r[2] = fmax(r[2], o[2]);
+    r[3] += o[3];
+    r[4] += o[4];
+    r[5] += o[5];
+    r[6] += o[6];
+  }
+  for (int k = 0; k < 7; k++) out[k] = r[k];
+}
+
+// Streaming copy (16-byte vectors), the HBM reference point for the roofline
+// and for scaling the FETCH_SIZE / WRITE_SIZE counters.
+__global__ void __launch_bounds__(256) hbm_copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src,
+                                                       uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) dst[i] = src[i];
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_rasterize(uint8_t* flags, int nx, int ny, int nz, double dx, const ObstacleDev* obs, int nobs,
+                            hipStream_t s) {
+  dim3 b(64, 4, 1), g((nx + 63) / 64, (ny + 3) / 4, nz);
+  hipLaunchKernelGGL(k_rasterize, g, b, 0, s, flags, nx, ny, nz, dx, obs, nobs);
+  return hipGetLastError();
+}
+
+template <class S>
+static hipError_t set_dist_t(void* f, uint64_t stride, int nx, int ny, const int* lo, const int* hi, double value,
+                             hipStream_t s) {
+  const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
+  dim3 b(64, 4, 1), g((ex + 63) / 64, (ey + 3) / 4, ez);
+  hipLaunchKernelGGL((k_set_dist<S>), g, b, 0, s, f, stride, nx, ny, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2],
+                     value);
+  return hipGetLastError();
+}
+
+hipError_t launch_set_distribution(int storage, void* f, uint64_t stride, int nx, int ny, const int* lo,
+                                   const int* hi, double value, hipStream_t s) {
+  switch (storage) {
+    case 0:
+      return set_dist_t<double>(f, stride, nx, ny, lo, hi, value, s);
+    case 1:
+      return set_dist_t<float>(f, stride, nx, ny, lo, hi, value, s);
+    case 2:
+      return set_dist_t<_Float16>(f, stride, nx, ny, lo, hi, value, s);
+    case 3:
+      return set_dist_t<bf16s>(f, stride, nx, ny, lo, hi, value, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <class S>
+static hipError_t conv_t(bool to_store, void* f, double* tmp, uint32_t n, int q, hipStream_t s) {
+  dim3 b(256), g((n + 255) / 256);
+  if (to_store)
+    hipLaunchKernelGGL((k_from_f64<S>), g, b, 0, s, f, (const double*)tmp, n, q);
+  else
+    hipLaunchKernelGGL((k_to_f64<S>), g, b, 0, s, tmp, (const void*)f, n, q);
+  return hipGetLastError();
+}
+
+hipError_t launch_convert(int storage, bool to_store, void* fplane, double* tmp, uint32_t n, int q, hipStream_t s) {
+  switch (storage) {
+    case 0:
+      return conv_t<double>(to_store, fplane, tmp, n, q, s);
+    case 1:
+      return conv_t<float>(to_store, fplane, tmp, n, q, s);
+    case 2:
+      return conv_t<_Float16>(to_store, fplane, tmp, n, q, s);
+    case 3:
+      return conv_t<bf16s>(to_store, fplane, tmp, n, q, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_macro(int storage, const void* f, uint64_t stride, uint32_t N, const uint8_t* flags, double* m0,
+                        double* m1, hipStream_t s) {
+  dim3 b(256), g((N + 255) / 256);
+  switch (storage) {
+    case 0:
+      hipLaunchKernelGGL((k_macro<double>), g, b, 0, s, f, stride, N, flags, m0, m1);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_macro<float>), g, b, 0, s, f, stride, N, flags, m0, m1);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_macro<_Float16>), g, b, 0, s, f, stride, N, flags, m0, m1);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_macro<bf16s>), g, b, 0, s, f, stride, N, flags, m0, m1);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s) {
+  const uint64_t n = bytes / 16;
+  uint64_t blocks = (n + 255) / 256;
+  if (blocks > 256 * 1024) blocks = 256 * 1024;
+  hipLaunchKernelGGL(hbm_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (uint4*)dst, (const uint4*)src, n);
+  return hipGetLastError();
+}
+
+int reduce_parts(uint32_t N) {
+  int parts = (int)((N + RED_BLOCK - 1) / RED_BLOCK);
+  return parts > 1024 ? 1024 : (parts < 1 ? 1 : parts);
+}
+
+hipError_t launch_reduce(const double* m0, const double* m1, const uint8_t* flags, uint32_t N, double* part,
+                         double* out, hipStream_t s) {
+  const int parts = reduce_parts(N);
+  hipLaunchKernelGGL(k_reduce_partial, dim3(parts), dim3(RED_BLOCK), 0, s, m0, m1, flags, N, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(64), 0, s, (const double*)part, parts, out);
+  return hipGetLastError();
+}
+
+}  // namespace hlbm

@@ -1,0 +1,186 @@
+/*
+ * hlbm.h -- C ABI of the MI355X (gfx950) D3Q19 lattice Boltzmann engine.
+ *
+ * This is the drop-in boundary for hippoLBM's hot path.  In the reference each
+ * hot-path step is an Onika operator (C++ class + ADD_SLOT + execute()) that
+ * mutates LBMFields<19> in place; this library replaces the bodies of those
+ * operators.  A host-side operator layer (hippolbm_amd/csrc/operators.cpp, or
+ * any Onika plugin) keeps the reference's operator names
+ * and slots and calls the entry points below.
+ *
+ * Conventions
+ *   - every function returns HLBM_OK (0) or a negative status; no exception
+ *     crosses the ABI; hlbm_last_error() returns a message for the calling
+ *     thread's last failure;
+ *   - one host thread per domain handle at a time (thread-compatible);
+ *   - `stream` arguments are hipStream_t passed as void* (NULL = the legacy
+ *     default stream);
+ *   - host pointers are plain C arrays; the populations stay resident in HBM
+ *     between calls;
+ *   - "node layout": Real nodes only, x fastest, f[i*N + (z*ny + y)*nx + x],
+ *     N = nx*ny*nz;  "reference layout": hippoLBM's LBMFields<19> layout over
+ *     the All box with two ghost layers per side,
+ *     f[i*N_all + (z*Ly + y)*Lx + x], L = n + 4
+ *     (src/grid/include/hippoLBM/grid/field_view.hpp:56-72,
+ *      include/hippoLBM/core/box3d.hpp:127-132).
+ */
+#ifndef HLBM_H
+#define HLBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define HLBM_OK 0
+#define HLBM_ERR_ARG (-1)         /* invalid argument                           */
+#define HLBM_ERR_HIP (-2)         /* a HIP runtime call failed                  */
+#define HLBM_ERR_UNSUPPORTED (-3) /* operator combination the engine rejects    */
+#define HLBM_ERR_NOMEM (-4)       /* device allocation failed                   */
+#define HLBM_ERR_NODEVICE (-5)    /* no HIP device visible                      */
+
+/* ---- enums ------------------------------------------------------------ */
+enum hlbm_storage { HLBM_F64 = 0, HLBM_F32 = 1, HLBM_F16 = 2, HLBM_BF16 = 3 };
+enum hlbm_pattern { HLBM_PATTERN_AB = 0, HLBM_PATTERN_AA = 1 };
+enum hlbm_collision { HLBM_BGK = 0, HLBM_MRT = 1 };
+enum hlbm_layout { HLBM_LAYOUT_NODES = 0, HLBM_LAYOUT_REF = 1 };
+
+/* boundary operators, in the reference's operator-name vocabulary */
+enum hlbm_bc_kind {
+  HLBM_BC_PRE_BOUNCE_BACK = 1,   /* "pre_bounce_back"   src/bcs/pre_bounce_back.cu:89-118   */
+  HLBM_BC_POST_BOUNCE_BACK = 2,  /* "post_bounce_back"  src/bcs/post_bounce_back.cu:101-123 */
+  HLBM_BC_WALL_BOUNCE_BACK = 3,  /* "wall_bounce_back"  src/bcs/wall_bounce_back.cu:58-73   */
+  HLBM_BC_CAVITY_Z_0 = 4,        /* "cavity_z_0"        src/bcs/cavity.cu:66-99             */
+  HLBM_BC_CAVITY_Z_L = 5,        /* "cavity_z_l"        src/bcs/cavity.cu:66-99             */
+  HLBM_BC_LID_DRIVEN_CAVITY = 6, /* "lid_driven_cavity" src/bcs/lid_driven_cavity.cu:69-93  */
+  HLBM_BC_NEUMANN = 7            /* "neumann"           src/bcs/neuman.cu:69-122            */
+};
+/* operator lists of the compute loop (data/config/config_compute_loop.msp:28-36) */
+enum hlbm_bc_stage { HLBM_STAGE_PRE_STREAM = 0, HLBM_STAGE_POST_STREAM = 1, HLBM_STAGE_BOUNDARY = 2 };
+/* plane traversals (src/grid/include/hippoLBM/grid/grid_region.hpp:75-77) */
+enum hlbm_plane {
+  HLBM_PLAN_XY_0 = 0, HLBM_PLAN_XY_L = 1, HLBM_PLAN_XZ_0 = 2,
+  HLBM_PLAN_XZ_L = 3, HLBM_PLAN_YZ_0 = 4, HLBM_PLAN_YZ_L = 5
+};
+
+/* hlbm_step flags */
+#define HLBM_STEP_WRITE_MACRO 1u /* the last kernel of the call also writes rho/u (macro_variables) */
+
+/* ---- descriptors ------------------------------------------------------ */
+typedef struct hlbm_domain hlbm_domain;
+
+typedef struct {
+  int32_t nodes[3];    /* grid nodes per axis = cell_dims + 1 (src/grid/domain_lbm.cpp:68)     */
+  int32_t periodic[3]; /* periodic axes (domain slot `periodic`)                               */
+  int32_t storage;     /* enum hlbm_storage; HLBM_F64 is the bit-faithful drop-in              */
+  int32_t pattern;     /* enum hlbm_pattern                                                    */
+  int32_t collision;   /* enum hlbm_collision                                                  */
+  int32_t mrt_custom;  /* 0: hippoLBM's hard-coded MRT rates (mrt.hpp:43-47); 1: mrt_rates    */
+  double tau;          /* LBMParameters::tau_                                                  */
+  double fext[3];      /* LBMParameters::Fext_ (LBM units)                                     */
+  double dx;           /* grid spacing, used by obstacle rasterization                         */
+  double mrt_rates[10];/* s1, s2, s4, s9, s10, s13, s16, weps, wepsj, wxx                     */
+} hlbm_domain_desc;
+
+typedef struct {
+  int32_t kind;  /* enum hlbm_bc_kind                                                           */
+  int32_t stage; /* enum hlbm_bc_stage: which list of the compute loop the operator sits in    */
+  int32_t plane; /* enum hlbm_plane, for lid_driven_cavity / neumann (one entry per region)    */
+  int32_t pad;
+  double U[3];   /* velocity in LBM units for lid/neumann (already divided by celerity);
+                    raw operator value for cavity_z_* (hippoLBM applies no conversion there) */
+} hlbm_bc;
+
+typedef struct {
+  int32_t kind; /* 0 = ball (register_solid_ball), 1 = wall (register_solid_wall)              */
+  int32_t pad;
+  double p0[3]; /* ball: center; wall: bmin                                                    */
+  double p1[3]; /* wall: bmax                                                                  */
+  double radius;
+} hlbm_obstacle;
+
+/* ---- library ---------------------------------------------------------- */
+const char* hlbm_last_error(void);
+const char* hlbm_version(void);
+/* Number of visible HIP devices (0 on a host without GPU); never allocates. */
+int hlbm_device_count(int32_t* count);
+
+/* ---- domain lifetime: replaces `domain` + `build_traversal` + `define_grid_3dq19`
+ *      (src/grid/domain_lbm.cpp:66-110, src/grid/build_traversal.cpp:58-62,
+ *       src/grid/define_lbm_fields.cpp:58-86) for one single-rank subdomain.
+ *      Populations start at zero, flags at fluid, rho/u at zero. */
+int hlbm_domain_create(const hlbm_domain_desc* desc, hlbm_domain** out);
+int hlbm_domain_destroy(hlbm_domain* h);
+/* Device bytes held by the domain (populations + flags + macro + scratch). */
+int hlbm_domain_bytes(const hlbm_domain* h, uint64_t* bytes);
+
+/* ---- boundary-condition plan: the pre_stream_bcs / post_stream_bcs /
+ *      boundary_conditions operator lists, in list order.  Replaces the
+ *      execute() bodies of pre_bounce_back, post_bounce_back, wall_bounce_back,
+ *      cavity_z_0/l, lid_driven_cavity and neumann.  Call again to change it. */
+int hlbm_set_bcs(hlbm_domain* h, const hlbm_bc* list, int32_t n);
+/* Change tau / Fext (lbm_parameters) without recreating the domain. */
+int hlbm_set_params(hlbm_domain* h, double tau, const double fext[3]);
+
+/* ---- initial state ----------------------------------------------------- */
+/* init_obst: every node fluid (src/grid/init_obst.cu:54-60). */
+int hlbm_init_obst(hlbm_domain* h);
+/* update_obstacles for a list of balls/walls in obstacle-id order, rasterized
+ * on the device with the reference's rules (src/obstacle/update_obstacles.cu:32-71). */
+int hlbm_add_obstacles(hlbm_domain* h, const hlbm_obstacle* list, int32_t n);
+/* Obstacle field as int32 -1 (fluid) / -2 (wall), node or reference layout. */
+int hlbm_set_flags(hlbm_domain* h, const int32_t* obst, int32_t layout);
+int hlbm_get_flags(hlbm_domain* h, int32_t* obst, int32_t layout);
+/* set_distribution: f_i = value*w_i everywhere (bounds == NULL) or on the
+ * nodes of the box bounds = {xmin,ymin,zmin,xmax,ymax,zmax}
+ * (src/grid/set_distribution.cu:69-109). */
+int hlbm_set_distribution(hlbm_domain* h, double value, const double* bounds);
+/* Populations in FP64 (converted from the storage type; the shifted storage
+ * adds w_i back).  get writes the current state f^T as the reference holds it
+ * after T time steps (post-stream, post-BC); in reference layout the periodic
+ * ghost layers are filled as update_ghost would, other ghost entries are left
+ * untouched. */
+int hlbm_set_populations_f64(hlbm_domain* h, const double* f, int32_t layout);
+int hlbm_get_populations_f64(hlbm_domain* h, double* f, int32_t layout);
+
+/* ---- time stepping ------------------------------------------------------
+ * Advance nsteps reference time steps: macro_variables + collision +
+ * pre_stream_bcs + streaming (+ update_ghost) + post_stream_bcs +
+ * boundary_conditions (data/config/config_compute_loop.msp:28-36), fused into
+ * one pull kernel per step.  Asynchronous on `stream`. */
+int hlbm_step(hlbm_domain* h, int32_t nsteps, uint32_t flags, void* stream);
+/* Steps taken since creation / the last set of populations. */
+int hlbm_timestep(const hlbm_domain* h, int64_t* t);
+int hlbm_synchronize(hlbm_domain* h);
+
+/* ---- macroscopic fields and diagnostics -------------------------------- */
+/* macro_variables of the current state (src/collision/macro_variables.cu:71-90):
+ * fills the domain's rho/u fields; copies them out when rho/u are non-NULL
+ * (rho: N values, u: 3*N SoA, node or reference layout). */
+int hlbm_macro(hlbm_domain* h, double* rho, double* u, int32_t layout);
+/* Copy the domain's rho/u fields as they are (the m0_/m1_ slots). */
+int hlbm_get_macro(hlbm_domain* h, double* rho, double* u, int32_t layout);
+/* simulation_state over the rho/u fields (src/io/simulation_state.cu:73-98),
+ * FP64 on the device: out[0] = sum rho, out[1] = min |u|, out[2] = max |u|,
+ * out[3..5] = sum rho*u (momentum), out[6] = number of fluid nodes. */
+int hlbm_reduce_stats(hlbm_domain* h, double out[7]);
+
+/* update_ghost (src/grid/update_ghost.cu:66-73): on one GPU there is no halo;
+ * this fills the periodic ghost layers of a reference-layout FP64 population
+ * array from its Real nodes, on the host array given.  It exists so an Onika
+ * deck that calls update_ghost explicitly keeps working. */
+int hlbm_ghost_update_ref(const hlbm_domain* h, double* f_ref);
+
+/* ---- measurement helper (no counterpart in the reference) --------------
+ * Times `reps` streaming copies of `bytes` bytes between two scratch buffers
+ * it allocates and frees itself (kernel "hbm_copy_kernel"): the achievable
+ * HBM bandwidth and the calibration point for the HBM byte counters. */
+int hlbm_copy_bench(uint64_t bytes, int32_t reps, void* stream, double* ms_per_copy);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HLBM_H */

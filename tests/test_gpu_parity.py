@@ -1,0 +1,262 @@
+"""FP64 parity of the HIP engine (through the C ABI) with the CPU oracle.
+
+The FP64 kernel keeps the reference's evaluation order without contraction, so
+the bar is bit equality of every population on every fluid node, of rho/u and
+of the obstacle flags.  Cases: the seven regression decks, a seeded Couette,
+a porous MRT channel with rasterized balls and wall bounce-back, and
+operator-order variants.
+"""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from hippolbm_amd import _lib
+from hippolbm_amd import deck as dk
+from hippolbm_amd.engine import Domain, Simulation
+
+from conftest import DECKS
+
+pytestmark = pytest.mark.gpu
+
+DECK_FILES = sorted(glob.glob(os.path.join(DECKS, "regression_*.msp")))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def compare(sim, ref, fluid_only=False):
+    """Populations f^N and rho/u of the GPU simulation vs the oracle run."""
+    o = ref.o
+    fo = o.node_f()
+    fg = sim.dom.get_f()
+    obst = o.real(o.obst())
+    fluid = obst == -1
+    mask = fluid if fluid_only else np.ones_like(fluid)
+    diff = fg[:, mask] != fo[:, mask]
+    return int(diff.sum()), float(np.abs(fg[:, mask] - fo[:, mask]).max()), fluid
+
+
+@pytest.mark.parametrize("path", DECK_FILES, ids=[os.path.basename(p) for p in DECK_FILES])
+def test_regression_deck_bitwise(path):
+    d = dk.load_msp(path)
+    say("deck", os.path.basename(path))
+    ref = orc.OracleRun(d)
+    sim = Simulation(d, storage="f64")
+    # flags are bit-exact
+    assert np.array_equal(sim.dom.get_flags(), ref.o.real(ref.o.obst()))
+    steps = d.end_iteration
+    ref.step(steps)
+    sim.run(steps)
+    nbad, maxdiff, fluid = compare(sim, ref, fluid_only=True)
+    assert nbad == 0, f"{nbad} fluid populations differ, max |diff| {maxdiff}"
+    # walls too (no obstacle touches a periodic face in these decks)
+    nbad_all, maxdiff_all, _ = compare(sim, ref)
+    assert nbad_all == 0, f"{nbad_all} populations differ (walls), max {maxdiff_all}"
+    # macro fields written by the last step (macro_variables of f^{N-1}) and the checker
+    rho, u = sim.dom.get_macro()
+    assert np.array_equal(rho[fluid], ref.o.real(ref.o.m0())[fluid])
+    assert np.array_equal(u[:, fluid], ref.o.real(ref.o.m1())[:, fluid])
+    s = sim.state()
+    so = ref.final_state()
+    assert abs(s["sum_density"] - so[0]) < 1e-9
+    assert s["min_velocity_norm"] == so[1] and s["max_velocity_norm"] == so[2]
+    assert sim.check() == []
+    sim.dom.close()
+
+
+def _couette_deck(n, tau=0.8, U=0.05):
+    txt = f"""
+do_domain:
+  - domain:
+     bounds: [[0,0,0],[{n - 1},{n - 1},{n - 1}]]
+     cell_dims: [{n - 1},{n - 1},{n - 1}]
+     periodic: [true, true, false]
+set_lbm_parameters:
+  - lbm_parameters:
+     tau: {tau}
+     nuth: {(tau - 0.5) / 3}
+pre_stream_bcs:
+  - pre_bounce_back
+  - cavity_z_l:
+     U: [{U}, 0, 0]
+post_stream_bcs:
+  - post_bounce_back
+"""
+    return dk.loads_msp(txt, "couette")
+
+
+def seeded_equilibrium(nodes, seed, amp=1e-3):
+    """rho = 1 + U(-amp, amp) (splitmix64 -> 53-bit uniform), u = 0, f = f_eq."""
+    n = nodes[0] * nodes[1] * nodes[2]
+    state = np.uint64(seed)
+    out = np.empty(n)
+    with np.errstate(over="ignore"):
+        for k in range(n):
+            state = np.uint64(state + np.uint64(0x9E3779B97F4A7C15))
+            z = state
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+            out[k] = float(z >> np.uint64(11)) * (1.0 / 9007199254740992.0)
+    rho = 1.0 + (2.0 * out - 1.0) * amp
+    w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+    return (w[:, None] * rho[None, :]).reshape(19, nodes[2], nodes[1], nodes[0])
+
+
+def test_seeded_couette_bitwise():
+    d = _couette_deck(20)
+    f0 = seeded_equilibrium(d.nodes, 12345)
+    say("couette 20^3 seeded")
+    ref = orc.OracleRun(d, init_f=f0)
+    sim = Simulation(d, storage="f64", init_f=f0)
+    ref.step(150)
+    sim.run(150)
+    nbad, maxdiff, _ = compare(sim, ref)
+    assert nbad == 0, (nbad, maxdiff)
+    sim.dom.close()
+
+
+def _porous_deck(nx, ny, nz, nballs, seed, collision="mrt"):
+    rng = np.random.default_rng(seed)
+    lines = [f"""
+do_domain:
+  - domain:
+     bounds: [[0,0,0],[{nx - 1},{ny - 1},{nz - 1}]]
+     cell_dims: [{nx - 1},{ny - 1},{nz - 1}]
+     periodic: [false, true, false]
+set_lbm_parameters:
+  - lbm_parameters:
+     tau: 0.7
+     nuth: {0.2 / 3}
+     Fext: [2.0e-5, 0, 0]
+collision: {collision}
+pre_stream_bcs:
+  - pre_bounce_back
+  - wall_bounce_back
+post_stream_bcs:
+  - post_bounce_back
+set_obstacles:
+"""]
+    for k in range(nballs):
+        # balls stay off the periodic y faces: there the reference's wall_bounce_back
+        # reads ghost nodes that still hold the previous step's values
+        r = rng.uniform(2.0, 3.5)
+        c = (rng.uniform(0, nx - 1), rng.uniform(r + 2, ny - 3 - r), rng.uniform(0, nz - 1))
+        lines.append(f"  - register_solid_ball:\n     id: {k}\n     center: [{c[0]},{c[1]},{c[2]}]\n     radius: {r}\n")
+    return dk.loads_msp("".join(lines), "porous")
+
+
+@pytest.mark.parametrize("collision", ["mrt", "bgk"])
+def test_porous_channel_bitwise(collision):
+    d = _porous_deck(24, 20, 18, 8, 3, collision)
+    say("porous", collision)
+    ref = orc.OracleRun(d)
+    sim = Simulation(d, storage="f64")
+    assert np.array_equal(sim.dom.get_flags(), ref.o.real(ref.o.obst()))
+    assert (sim.dom.get_flags() == -2).sum() > 100
+    ref.step(60)
+    sim.run(60)
+    nbad, maxdiff, _ = compare(sim, ref, fluid_only=True)
+    assert nbad == 0, (nbad, maxdiff)
+    nbad, maxdiff, _ = compare(sim, ref)
+    assert nbad == 0, (nbad, maxdiff)
+    sim.dom.close()
+
+
+def test_operator_order_lid_before_post_bounce_back():
+    # lid_driven_cavity listed before post_bounce_back: post_bounce_back then
+    # overwrites the lid's incoming populations on the closed faces
+    txt = """
+do_domain:
+  - domain:
+     bounds: [[0,0,0],[11,9,13]]
+     cell_dims: [11, 9, 13]
+     periodic: [false, false, false]
+set_lbm_parameters:
+  - lbm_parameters:
+     tau: 0.9
+     nuth: 0.13333333333333333
+pre_stream_bcs:
+  - pre_bounce_back
+  - cavity_z_0:
+     U: [0.0, 0.02, 0]
+post_stream_bcs:
+  - lid_driven_cavity:
+     U: [0.03, 0.0, 0]
+     regions: [plan_xy_l, plan_yz_0]
+  - post_bounce_back
+boundary_conditions:
+  - neumann:
+     U: [0.0, 0.01, 0]
+     regions: [plan_xz_0]
+"""
+    d = dk.loads_msp(txt, "order")
+    ref = orc.OracleRun(d)
+    sim = Simulation(d, storage="f64")
+    ref.step(40)
+    sim.run(40)
+    nbad, maxdiff, _ = compare(sim, ref)
+    assert nbad == 0, (nbad, maxdiff)
+    sim.dom.close()
+
+
+def test_reference_layout_ghost_fill_matches_update_ghost():
+    d = _couette_deck(9)
+    f0 = seeded_equilibrium(d.nodes, 99)
+    ref = orc.OracleRun(d, init_f=f0)
+    sim = Simulation(d, storage="f64", init_f=f0)
+    ref.step(7)
+    sim.run(7)
+    ref.o.update_ghost()
+    fr = ref.o.f()
+    g = np.zeros_like(fr)
+    sim.dom.get_f(layout=_lib.LAYOUT_REF, out=g)
+    # Real nodes and the periodic (x, y) ghost layers of the Real z range
+    sl = (slice(None), slice(2, -2), slice(None), slice(None))
+    assert np.array_equal(g[sl], fr[sl])
+    sim.dom.close()
+
+
+def test_macro_variables_on_current_state():
+    d = _couette_deck(12)
+    f0 = seeded_equilibrium(d.nodes, 5)
+    ref = orc.OracleRun(d, init_f=f0)
+    sim = Simulation(d, storage="f64", init_f=f0)
+    ref.step(11)
+    sim.run(11)
+    ref.o.macro_variables()
+    rho, u = sim.dom.macro()
+    assert np.array_equal(rho, ref.o.real(ref.o.m0()))
+    assert np.array_equal(u, ref.o.real(ref.o.m1()))
+    sim.dom.close()
+
+
+def test_step_chunking_is_invariant():
+    # step(a) + step(b) == step(a + b), also across population read-backs
+    d = _couette_deck(10)
+    f0 = seeded_equilibrium(d.nodes, 77)
+    a = Simulation(d, storage="f64", init_f=f0)
+    b = Simulation(d, storage="f64", init_f=f0)
+    a.run(30)
+    for k in (1, 4, 10, 15):
+        b.run(k)
+        b.dom.get_f()
+    assert np.array_equal(a.dom.get_f(), b.dom.get_f())
+    assert a.dom.timestep() == b.dom.timestep() == 30
+    a.dom.close()
+    b.dom.close()
+
+
+def test_unsupported_operator_placement_is_rejected():
+    dom = Domain((8, 8, 8), (True, True, False), 0.8)
+    with pytest.raises(_lib.HLBMError):
+        dom.set_bcs([("pre_stream_bcs", "neumann", {"U": (0, 0, 0), "regions": ["plan_xy_0"]})])
+    with pytest.raises(_lib.HLBMError):
+        dom.set_flags(np.full(8 * 8 * 8, 3, dtype=np.int32))
+    dom.close()

@@ -1,0 +1,65 @@
+"""Reduced-precision forms (FP32 compute on shifted f - w stored as FP32, FP16,
+BF16) against the FP64 engine on the same inputs.  Tolerances are the error
+bounds this build states per storage type."""
+import numpy as np
+import pytest
+
+from hippolbm_amd import deck as dk
+from hippolbm_amd.engine import Simulation
+
+pytestmark = pytest.mark.gpu
+
+# relative L2 error of u and max |rho error| after the run, per storage type
+TOL = {"f32": (1e-5, 1e-6), "f16": (2e-2, 1e-3), "bf16": (1e-1, 1e-2)}
+
+
+def deck(n, collision="bgk"):
+    return dk.loads_msp(f"""
+do_domain:
+  - domain:
+     bounds: [[0,0,0],[{n - 1},{n - 1},{n - 1}]]
+     cell_dims: [{n - 1},{n - 1},{n - 1}]
+     periodic: [true, true, false]
+set_lbm_parameters:
+  - lbm_parameters:
+     tau: 0.8
+     nuth: 0.1
+collision: {collision}
+pre_stream_bcs:
+  - pre_bounce_back
+  - cavity_z_l:
+     U: [0.05, 0, 0]
+post_stream_bcs:
+  - post_bounce_back
+""", "couette")
+
+
+@pytest.mark.parametrize("collision", ["bgk", "mrt"])
+@pytest.mark.parametrize("storage", ["f32", "f16", "bf16"])
+def test_shifted_storage_error_vs_f64(storage, collision):
+    d = deck(32, collision)
+    ref = Simulation(d, storage="f64")
+    low = Simulation(d, storage=storage)
+    ref.run(400)
+    low.run(400)
+    r0, u0 = ref.dom.macro()
+    r1, u1 = low.dom.macro()
+    rel_l2 = np.linalg.norm(u1 - u0) / np.linalg.norm(u0)
+    rho_err = np.abs(r1 - r0).max()
+    print(f"{storage} {collision}: rel L2(u) = {rel_l2:.3e}, max|drho| = {rho_err:.3e}", flush=True)
+    tu, tr = TOL[storage]
+    assert rel_l2 < tu and rho_err < tr
+    ref.dom.close()
+    low.dom.close()
+
+
+def test_mass_conservation_f16():
+    d = deck(24)
+    s = Simulation(d, storage="f16")
+    s.dom.macro()
+    m0 = s.state()["sum_density"]
+    s.run(300)
+    s.dom.macro()
+    m1 = s.state()["sum_density"]
+    assert abs(m1 - m0) / m0 < 1e-4
+    s.dom.close()

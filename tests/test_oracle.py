@@ -1,0 +1,373 @@
+"""Pins for the CPU oracle (oracle/hlbm_oracle.cpp): the reference's own
+regression values and closed forms that a plausible mistake would break.
+
+The oracle is the checker for the GPU engine, so it is pinned here against
+data that does not come from itself:
+  * regression-test/*.msp (sum rho, min/max |u| after 100 steps);
+  * D3Q19 moment identities;
+  * BGK: equilibrium is a fixed point, mass/momentum conserved, force adds rho*F;
+  * MRT under the LBGK parametrisation (mrt.hpp:48-50) equals BGK;
+  * swap streaming equals a cyclic shift (periodic) plus on-node bounce-back
+    (non-periodic faces), computed here with numpy.roll;
+  * Zou/He reproduces the prescribed wall velocity, the lid sets u = U;
+  * rasterization wall counts, ball counts by brute force;
+  * analytic Poiseuille and Couette profiles.
+"""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from hippolbm_amd import deck as dk
+
+from conftest import DECKS
+
+W = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+C = np.array([[0, 0, 0], [1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, 1], [0, 0, -1], [1, 1, 0], [-1, -1, 0],
+              [1, -1, 0], [-1, 1, 0], [1, 0, 1], [-1, 0, -1], [1, 0, -1], [-1, 0, 1], [0, 1, 1], [0, -1, -1],
+              [0, 1, -1], [0, -1, 1]])
+OPP = [0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17]
+
+
+def feq(rho, u):
+    """Second-order D3Q19 equilibrium, arrays rho (...), u (3, ...)."""
+    cu = np.einsum("qd,d...->q...", C, u)
+    usq = (u * u).sum(0)
+    return W.reshape((19,) + (1,) * rho.ndim) * rho * (1 + 3 * cu + 4.5 * cu * cu - 1.5 * usq)
+
+
+def moments(f):
+    rho = f.sum(0)
+    j = np.einsum("qd,q...->d...", C, f)
+    return rho, j
+
+
+# ---------------------------------------------------------------------------
+# regression decks (the only reference-held pins)
+# ---------------------------------------------------------------------------
+DECK_FILES = sorted(glob.glob(os.path.join(DECKS, "regression_*.msp")))
+
+
+def test_seven_regression_decks_present():
+    assert len(DECK_FILES) == 7
+
+
+@pytest.mark.parametrize("path", DECK_FILES, ids=[os.path.basename(p) for p in DECK_FILES])
+def test_regression_pin(path):
+    d = dk.load_msp(path)
+    r = orc.OracleRun(d)
+    r.step(d.end_iteration)
+    s = r.final_state()
+    c = d.checker
+    assert abs(s[0] - c["density"]) <= c["tol"], (s, c)
+    assert s[1] >= c.get("vmin", 0.0) - 1e-15
+    assert s[2] <= c["vmax"] + 1e-15
+
+
+def test_wall_counts_pin_rasterization_rules():
+    # strict AABB (obstacle.hpp:13-17) + int() truncation + position (local+offset)*dx:
+    # the regression sums 29124 and 32243 follow from these node counts
+    counts = {}
+    for name in ("regression_lbm_cavite_wall.msp", "regression_lbm_diff_pressure_complex_test.msp"):
+        d = dk.load_msp(os.path.join(DECKS, name))
+        o = orc.Oracle(d.nodes, d.periodic, d.dx)
+        o.init_obst()
+        for ob in d.obstacles_in_id_order():
+            o.update_obstacle("wall", ob["bounds"][0], ob["bounds"][1])
+        counts[name] = int((o.real(o.obst()) == -2).sum())
+    assert counts["regression_lbm_cavite_wall.msp"] == 667  # 29791 - 29124
+    # 32243 = 1.5*w-sum over x<=7 fluid + 1 elsewhere; the wall count fixes it
+    assert counts["regression_lbm_diff_pressure_complex_test.msp"] == 1392
+
+
+def test_ball_rasterization_brute_force():
+    rng = np.random.default_rng(7)
+    n = (21, 17, 19)
+    dx = 0.05
+    o = orc.Oracle(n, (False, True, False), dx)
+    o.init_obst()
+    balls = [(rng.uniform(0, 1.0, 3), rng.uniform(0.08, 0.3)) for _ in range(6)]
+    for c, r in balls:
+        o.update_obstacle("ball", c, r=r)
+    z, y, x = np.meshgrid(np.arange(n[2]), np.arange(n[1]), np.arange(n[0]), indexing="ij")
+    solid = np.zeros(x.shape, bool)
+    for c, r in balls:
+        d2 = (x * dx - c[0]) ** 2 + (y * dx - c[1]) ** 2 + (z * dx - c[2]) ** 2
+        solid |= d2 <= r * r
+    got = o.real(o.obst()) == -2
+    # brute force differs only where |p-c|^2 is within rounding of r^2
+    assert (got != solid).sum() <= 2
+    assert got.sum() > 50
+
+
+# ---------------------------------------------------------------------------
+# lattice and collision closed forms
+# ---------------------------------------------------------------------------
+def _oracle_with(f, periodic=(True, True, True)):
+    n = f.shape[1:][::-1]
+    o = orc.Oracle(n, periodic, 1.0)
+    o.init_obst()
+    o.set_node_f(f)
+    return o
+
+
+def test_lattice_moments_from_set_distribution():
+    o = orc.Oracle((3, 3, 3), (True, True, True), 1.0)
+    o.init_obst()
+    o.set_distribution(1.0)
+    f = o.node_f()[:, 0, 0, 0]
+    assert np.allclose(f, W, rtol=0, atol=0)
+    assert abs(f.sum() - 1) < 1e-15
+    assert np.allclose(C.T @ f, 0, atol=1e-17)
+    assert np.allclose(np.einsum("q,qa,qb->ab", f, C, C), np.eye(3) / 3, atol=1e-16)
+    for i in range(19):
+        assert (C[OPP[i]] == -C[i]).all()
+
+
+def test_bgk_fixed_point_and_conservation():
+    rng = np.random.default_rng(1)
+    n = (4, 5, 6)
+    rho = 1 + rng.uniform(-0.01, 0.01, n[::-1])
+    u = rng.uniform(-0.05, 0.05, (3,) + n[::-1])
+    fe = feq(rho, u)
+    o = _oracle_with(fe)
+    o.macro_variables()
+    o.bgk(0.8)
+    assert np.abs(o.node_f() - fe).max() < 2e-16
+    f = fe + rng.uniform(-1e-3, 1e-3, fe.shape)
+    o = _oracle_with(f)
+    o.macro_variables()
+    o.bgk(0.7)
+    r0, j0 = moments(f)
+    r1, j1 = moments(o.node_f())
+    assert np.abs(r1 - r0).max() < 1e-14
+    assert np.abs(j1 - j0).max() < 1e-14
+    # force: sum_i c_i * 3 rho w_i (c_i.F) = rho F
+    F = np.array([1e-4, -2e-4, 3e-4])
+    o = _oracle_with(f)
+    o.macro_variables()
+    o.bgk(0.7, F)
+    r2, j2 = moments(o.node_f())
+    assert np.abs(j2 - j0 - r0 * F.reshape(3, 1, 1, 1)).max() < 1e-14
+    # relaxation towards equilibrium: f' - feq = (1 - 1/tau)(f - feq)
+    o = _oracle_with(f)
+    o.macro_variables()
+    o.bgk(0.7)
+    rr, jj = moments(f)
+    fe2 = feq(rr, jj / rr)
+    assert np.abs((o.node_f() - fe2) - (1 - 1 / 0.7) * (f - fe2)).max() < 1e-15
+
+
+def test_mrt_equals_bgk_under_lbgk_rates():
+    # mrt.hpp:48-50: s_k = 1/tau for all k, weps = 3, wepsj = -11/2, wxx = -1/2
+    rng = np.random.default_rng(2)
+    n = (5, 4, 3)
+    tau = 0.73
+    s = 1 / tau
+    rates = [s, s, s, s, s, s, s, 3.0, -11.0 / 2, -0.5]
+    rho = 1 + rng.uniform(-0.01, 0.01, n[::-1])
+    u = rng.uniform(-0.05, 0.05, (3,) + n[::-1])
+    f = feq(rho, u) + rng.uniform(-1e-3, 1e-3, (19,) + n[::-1])
+    a = _oracle_with(f)
+    a.macro_variables()
+    a.bgk(tau)
+    b = _oracle_with(f)
+    b.macro_variables()
+    b.mrt(tau, rates=rates)
+    # identical up to the O(u^3) terms BGK keeps with rho0 = 1 vs rho: compare on rho=1 exactly
+    rho1 = np.ones(n[::-1])
+    f1 = feq(rho1, u)
+    f1 = f1 + rng.uniform(-1e-4, 1e-4, f1.shape)
+    f1 = f1 - (f1.sum(0) - 1)[None] * W.reshape(19, 1, 1, 1)
+    a = _oracle_with(f1)
+    a.macro_variables()
+    a.bgk(tau)
+    b = _oracle_with(f1)
+    b.macro_variables()
+    b.mrt(tau, rates=rates)
+    assert np.abs(a.node_f() - b.node_f()).max() < 1e-13
+
+
+def test_mrt_conserves_mass_and_momentum():
+    rng = np.random.default_rng(3)
+    n = (4, 4, 4)
+    f = feq(1 + rng.uniform(-0.01, 0.01, n), rng.uniform(-0.05, 0.05, (3,) + n)) + rng.uniform(-1e-3, 1e-3,
+                                                                                              (19,) + n)
+    o = _oracle_with(f)
+    o.macro_variables()
+    o.mrt(0.6)
+    r0, j0 = moments(f)
+    r1, j1 = moments(o.node_f())
+    assert np.abs(r1 - r0).max() < 1e-14
+    assert np.abs(j1 - j0).max() < 1e-14
+
+
+# ---------------------------------------------------------------------------
+# streaming: cyclic shift + on-node bounce-back, built with numpy
+# ---------------------------------------------------------------------------
+def stream_numpy(f, periodic):
+    out = np.empty_like(f)
+    nz, ny, nx = f.shape[1:]
+    z, y, x = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    for i in range(19):
+        sx, sy, sz = x - C[i, 0], y - C[i, 1], z - C[i, 2]
+        bb = np.zeros(x.shape, bool)
+        for s, n, p in ((sx, nx, periodic[0]), (sy, ny, periodic[1]), (sz, nz, periodic[2])):
+            if p:
+                s %= n
+            else:
+                bb |= (s < 0) | (s >= n)
+        src = f[i, np.clip(sz, 0, nz - 1), np.clip(sy, 0, ny - 1), np.clip(sx, 0, nx - 1)]
+        out[i] = np.where(bb, f[OPP[i]], src)
+    return out
+
+
+@pytest.mark.parametrize("periodic", [(True, True, True), (True, True, False), (False, True, False),
+                                      (False, False, False)])
+def test_swap_streaming_is_shift_plus_bounce_back(periodic):
+    rng = np.random.default_rng(4)
+    f = rng.uniform(0, 1, (19, 5, 6, 7))
+    o = _oracle_with(f, periodic)
+    o.streaming()
+    assert np.array_equal(o.node_f(), stream_numpy(f, periodic))
+
+
+def test_wall_bounce_back_is_halfway_bounce_back():
+    rng = np.random.default_rng(5)
+    n = (8, 7, 6)
+    f = rng.uniform(0, 1, (19,) + n[::-1])
+    o = _oracle_with(f, (False, False, False))
+    obst = o.obst()
+    solid = np.zeros(n[::-1], bool)
+    solid[2:4, 3:5, 2:5] = True
+    o.real(obst)[...] = np.where(solid, -2, -1)
+    o.wall_bounce_back()
+    o.streaming()
+    got = o.node_f()
+    ref = stream_numpy(f, (False, False, False))
+    # fluid x whose upstream x - c_i is a wall receives its own f*_opp(i)
+    nz, ny, nx = n[::-1]
+    for i in range(1, 19):
+        for (zz, yy, xx) in zip(*np.nonzero(~solid)):
+            s = (zz - C[i, 2], yy - C[i, 1], xx - C[i, 0])
+            if 0 <= s[0] < nz and 0 <= s[1] < ny and 0 <= s[2] < nx and solid[s]:
+                ref[i, zz, yy, xx] = f[OPP[i], zz, yy, xx]
+    fl = ~solid
+    assert np.array_equal(got[:, fl], ref[:, fl])
+
+
+# ---------------------------------------------------------------------------
+# boundary operators
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("plane,axis,side", [("plan_xy_0", 2, 0), ("plan_xy_l", 2, 1), ("plan_xz_0", 1, 0),
+                                             ("plan_xz_l", 1, 1), ("plan_yz_0", 0, 0), ("plan_yz_l", 0, 1)])
+def test_zou_he_reproduces_wall_velocity(plane, axis, side):
+    rng = np.random.default_rng(6)
+    n = (6, 5, 7)
+    rho = 1 + rng.uniform(-0.01, 0.01, n[::-1])
+    u = rng.uniform(-0.02, 0.02, (3,) + n[::-1])
+    f = feq(rho, u) + rng.uniform(-1e-4, 1e-4, (19,) + n[::-1])
+    o = _oracle_with(f, (False, False, False))
+    U = np.array([0.013, -0.007, 0.004])
+    o.neumann(plane, U)
+    g = o.node_f()
+    sl = [slice(None)] * 3
+    sl[2 - axis] = -1 if side else 0
+    r, j = moments(g[(slice(None),) + tuple(sl)])
+    assert np.abs(j / r - U.reshape(3, 1, 1)).max() < 1e-13
+
+
+def test_lid_driven_cavity_sets_equilibrium_at_local_density():
+    rng = np.random.default_rng(8)
+    n = (5, 5, 5)
+    f = feq(1 + rng.uniform(-0.01, 0.01, n), np.zeros((3,) + n)) + rng.uniform(-1e-3, 1e-3, (19,) + n)
+    o = _oracle_with(f, (False, True, False))
+    U = np.array([0.05, 0.0, 0.01])
+    o.lid_driven_cavity("plan_xy_l", U)
+    g = o.node_f()[:, -1]
+    r0 = f[:, -1].sum(0)
+    assert np.abs(g - feq(r0, np.broadcast_to(U.reshape(3, 1, 1), (3,) + r0.shape))).max() < 1e-15
+
+
+def test_cavity_coefficients_closed_form():
+    U = (0.0, 0.1, 0.0)
+    L = 31
+    up = np.array(U) * (1 + 0.5 / (L - 1))
+    for side, dirs in ((0, [5, 14, 11, 18, 15]), (1, [6, 13, 12, 17, 16])):
+        got = orc.cavity_coeff(side, L, U)
+        want = [6 * W[q] * (C[q] @ up) for q in dirs]
+        assert np.allclose(got, want, rtol=0, atol=1e-17)
+
+
+def test_pre_post_bounce_back_equal_implicit_bounce_back():
+    rng = np.random.default_rng(9)
+    f = rng.uniform(0, 1, (19, 5, 6, 7))
+    a = _oracle_with(f, (True, False, False))
+    a.streaming()
+    b = _oracle_with(f, (True, False, False))
+    b.pre_bounce_back()
+    b.streaming()
+    b.post_bounce_back()
+    assert np.array_equal(a.node_f(), b.node_f())
+
+
+# ---------------------------------------------------------------------------
+# analytic profiles
+# ---------------------------------------------------------------------------
+def test_poiseuille_steady_profile():
+    # periodic x,y; no-slip (Zou/He U=0) on both z planes; body force F (LBM units)
+    nz = 17
+    tau = 1.0
+    nu = (tau - 0.5) / 3
+    F = 1e-6
+    o = orc.Oracle((3, 3, nz), (True, True, False), 1.0)
+    o.init_obst()
+    o.set_distribution(1.0)
+    for _ in range(6000):
+        o.macro_variables()
+        o.bgk(tau, (F, 0.0, 0.0))
+        o.streaming()
+        o.neumann("plan_xy_0", (0.0, 0.0, 0.0))
+        o.neumann("plan_xy_l", (0.0, 0.0, 0.0))
+    o.macro_variables()
+    ux = o.real(o.m1())[0, :, 1, 1]
+    z = np.arange(nz)
+    H = nz - 1
+    ana = F / (2 * nu) * z * (H - z)
+    assert np.abs(ux - ana).max() / ana.max() < 0.02
+
+
+def test_couette_linear_profile():
+    nz = 12
+    tau = 0.9
+    U = 0.01
+    o = orc.Oracle((3, 3, nz), (True, True, False), 1.0)
+    o.init_obst()
+    o.set_distribution(1.0)
+    for _ in range(4000):
+        o.macro_variables()
+        o.bgk(tau)
+        o.streaming()
+        o.neumann("plan_xy_l", (U, 0.0, 0.0))
+        o.neumann("plan_xy_0", (0.0, 0.0, 0.0))
+    o.macro_variables()
+    ux = o.real(o.m1())[0, :, 1, 1]
+    ana = U * np.arange(nz) / (nz - 1)
+    assert np.abs(ux - ana).max() < 2e-5
+
+
+def test_lbm_parameters_closed_forms():
+    # lbm_parameters.hpp:61-82: the three ways to define tau
+    dx = 0.1 / 30
+    p = orc.lbm_parameters(dx, nuth=1e-4, celerity=1.0)
+    assert abs(p["tau"] - (3 * 1e-4 * dx / dx ** 2 + 0.5)) < 1e-14
+    p = orc.lbm_parameters(dx, nuth=1e-4, tau=0.8)
+    assert abs(p["nu"] - 0.3 / 3) < 1e-14 and abs(p["celerity"] - dx / p["dtLB"]) < 1e-12
+    p = orc.lbm_parameters(dx, nuth=1e-3, dt=1e-3, Fext=(1.0, 0, 0))
+    assert abs(p["tau"] - (3 * 1e-3 * 1e-3 / dx ** 2 + 0.5)) < 1e-12
+    assert abs(p["Fext"][0] - 1e-3 ** 2 / dx) < 1e-15
+    d = dk.load_msp(os.path.join(DECKS, "regression_lbm_poiseuille.msp"))
+    po = orc.lbm_parameters(d.dx, nuth=d.lbm["nuth"], celerity=d.lbm["celerity"], Fext=d.lbm["Fext"])
+    assert po["tau"] == d.params["tau"] and po["Fext"] == d.params["Fext"]
