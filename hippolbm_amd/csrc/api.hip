@@ -254,6 +254,9 @@ int hlbm_set_bcs(hlbm_domain* h, const hlbm_bc* list, int32_t n) {
     } else if (b.stage == HLBM_STAGE_POST_STREAM || b.stage == HLBM_STAGE_BOUNDARY) {
       switch (b.kind) {
         case HLBM_BC_POST_BOUNCE_BACK:
+          // post_bounce_back first in the post-stream lists: the restored values
+          // (bounce-back + moving wall) are what the gather already produces
+          if (!post_bb && P.nprog == 0) P.cav_gather = 1;
           post_bb = true;
           // The gather already applied on-node bounce-back (+ moving wall); a
           // post_bounce_back re-applies it only after an operator changed the node.

@@ -59,6 +59,7 @@ struct StepParams {
   int32_t wall_bb;      // wall_bounce_back active
   int32_t plane_bb;     // pre_bounce_back + post_bounce_back active
   int32_t cav_on[2];    // cavity on the low / high z face
+  int32_t cav_gather;   // post_bounce_back is the first post-stream operator: the gather adds the cavity term
   int32_t nprog;
   int32_t prog_planes;  // OR of plane bits used by the program
   int32_t mrt_custom;
@@ -571,7 +572,7 @@ __device__ __forceinline__ void process_node(const StepParams& p, int x, int y, 
         C v = O::ld(fin, (size_t)OPP(i) * st + idx);
         if constexpr (CZ(i) != 0) {
           constexpr int side = CZ(i) == 1 ? 0 : 1;
-          if (sz < 0 && p.cav_on[side]) v += C(p.cav[side][cavity_slot(i)]);
+          if (sz < 0 && p.cav_gather && p.cav_on[side]) v += C(p.cav[side][cavity_slot(i)]);
         }
         f[i] = v;
       } else {

@@ -64,7 +64,8 @@ def test_regression_deck_bitwise(path):
     assert np.array_equal(u[:, fluid], ref.o.real(ref.o.m1())[:, fluid])
     s = sim.state()
     so = ref.final_state()
-    assert abs(s["sum_density"] - so[0]) < 1e-9
+    # sum order differs (sequential on the CPU, tree on the GPU): relative 1e-12
+    assert abs(s["sum_density"] - so[0]) <= 1e-12 * abs(so[0])
     assert s["min_velocity_norm"] == so[1] and s["max_velocity_norm"] == so[2]
     assert sim.check() == []
     sim.dom.close()
