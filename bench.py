@@ -137,11 +137,16 @@ def main():
     ap.add_argument("--dtypes", default="f64,f32,f16,bf16")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", default=None, help="run a single storage type (profiling)")
+    ap.add_argument("--copy", action="store_true", help="run only the streaming-copy reference kernel (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(the domain-decomposed multi-GPU path is a design note, not code)", file=sys.stderr)
         return 2
+    if args.copy:
+        print(json.dumps({"copy_kernel_gbs": copy_bandwidth(), "bytes_read_per_launch": 4 << 30,
+                          "bytes_written_per_launch": 4 << 30}), flush=True)
+        return 0
     if args.only:
         r = measure(args.size, args.only, args.steps, args.warmup)
         print(json.dumps({"storage": args.only, **r}), flush=True)

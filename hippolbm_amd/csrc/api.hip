@@ -58,13 +58,21 @@ struct hlbm_domain {
   double* tmp = nullptr;  // one FP64 population plane, conversions
   bool any_wall = false;
   StepParams plan{};
-  dim3 block{64, 4, 1};
+  dim3 block{128, 2, 1};
+  int xcd_remap = 1;
+  int npt = 1;
   int device = 0;
   uint64_t bytes = 0;
 };
 
+static unsigned tiles_y(const hlbm_domain* h) {
+  const unsigned rows = h->block.y * h->npt;
+  return (h->ny + rows - 1) / rows;
+}
+
 static dim3 grid_for(const hlbm_domain* h) {
-  return dim3((h->nx + h->block.x - 1) / h->block.x, (h->ny + h->block.y - 1) / h->block.y, h->nz);
+  const unsigned gx = (h->nx + h->block.x - 1) / h->block.x;
+  return dim3(gx * tiles_y(h) * h->nz, 1, 1);
 }
 
 static StepParams base_params(hlbm_domain* h) {
@@ -83,6 +91,10 @@ static StepParams base_params(hlbm_domain* h) {
   // c_i . F in the reference's order: ex*Fx + ey*Fy + ez*Fz (bgk.hpp:57, mrt.hpp:188)
   for (int i = 0; i < Q; i++) p.ef[i] = CX(i) * h->d.fext[0] + CY(i) * h->d.fext[1] + CZ(i) * h->d.fext[2];
   p.mrt_custom = h->d.mrt_custom;
+  p.gx = (int)((h->nx + h->block.x - 1) / h->block.x);
+  p.gy = (int)tiles_y(h);
+  p.gz = h->nz;
+  p.xcd_remap = h->xcd_remap;
   for (int k = 0; k < 10; k++) p.mrt[k] = h->d.mrt_rates[k];
   if (!h->any_wall) p.wall_bb = 0;
   return p;
@@ -96,9 +108,9 @@ static int launch(hlbm_domain* h, int mode, bool write_macro, hipStream_t s) {
   hipError_t e;
   const int coll = h->d.collision == HLBM_MRT ? COLL_MRT : COLL_BGK;
   if (h->d.storage == HLBM_F64)
-    e = launch_step_f64(coll, mode, grid_for(h), h->block, p, s);
+    e = launch_step_f64(coll, mode, h->npt, grid_for(h), h->block, p, s);
   else
-    e = launch_step_lowp(h->d.storage, coll, mode, grid_for(h), h->block, p, s);
+    e = launch_step_lowp(h->d.storage, coll, mode, h->npt, grid_for(h), h->block, p, s);
   if (e != hipSuccess) return fail(HLBM_ERR_HIP, std::string("step kernel launch: ") + hipGetErrorString(e));
   h->cur ^= 1;
   return HLBM_OK;
@@ -151,6 +163,9 @@ int hlbm_domain_create(const hlbm_domain_desc* desc, hlbm_domain** out) {
   h->esize = elem_size(desc->storage);
   const uint64_t align = 256 / h->esize;
   h->stride = (N64 + align - 1) / align * align;
+  if (const char* pad = getenv("HLBM_PAD")) h->stride += (uint64_t)strtoull(pad, nullptr, 10);
+  if (const char* xr = getenv("HLBM_XCD_REMAP")) h->xcd_remap = atoi(xr);
+  if (const char* np = getenv("HLBM_NPT")) h->npt = atoi(np) == 2 ? 2 : 1;
   (void)hipGetDevice(&h->device);
   const char* blk = getenv("HLBM_BLOCK");
   if (blk) {

@@ -20,9 +20,9 @@ inline unsigned boundary_blocks(const StepParams& p) {
 }
 
 // FP64 storage, FP64 compute, no contraction (step_f64.hip)
-hipError_t launch_step_f64(int coll, int mode, dim3 grid, dim3 block, const StepParams& p, hipStream_t s);
+hipError_t launch_step_f64(int coll, int mode, int npt, dim3 grid, dim3 block, const StepParams& p, hipStream_t s);
 // FP32 compute on shifted FP32/FP16/BF16 storage (step_lowp.hip); storage = hlbm_storage
-hipError_t launch_step_lowp(int storage, int coll, int mode, dim3 grid, dim3 block, const StepParams& p,
+hipError_t launch_step_lowp(int storage, int coll, int mode, int npt, dim3 grid, dim3 block, const StepParams& p,
                             hipStream_t s);
 
 }  // namespace hlbm

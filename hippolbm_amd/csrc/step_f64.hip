@@ -5,24 +5,28 @@
 
 namespace hlbm {
 
-template <int COLL, int MODE>
+template <int COLL, int MODE, int NPT>
 static hipError_t go(dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
-  hipLaunchKernelGGL((step_kernel<double, COLL, MODE>), grid, block, 0, s, p);
+  hipLaunchKernelGGL((step_kernel<double, COLL, MODE, NPT>), grid, block, 0, s, p);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || MODE == MODE_COLLIDE || !p.prog_planes) return e;
   hipLaunchKernelGGL((boundary_kernel<double, COLL, MODE>), dim3(boundary_blocks(p)), dim3(256), 0, s, p);
   return hipGetLastError();
 }
 
-hipError_t launch_step_f64(int coll, int mode, dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
+template <int COLL, int MODE>
+static hipError_t go_npt(int npt, dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
+  return npt == 2 ? go<COLL, MODE, 2>(grid, block, p, s) : go<COLL, MODE, 1>(grid, block, p, s);
+}
+
+hipError_t launch_step_f64(int coll, int mode, int npt, dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
+  if (mode == MODE_STREAM) return go_npt<COLL_BGK, MODE_STREAM>(npt, grid, block, p, s);  // no collision
   if (coll == COLL_BGK) {
-    if (mode == MODE_FUSED) return go<COLL_BGK, MODE_FUSED>(grid, block, p, s);
-    if (mode == MODE_COLLIDE) return go<COLL_BGK, MODE_COLLIDE>(grid, block, p, s);
-    return go<COLL_BGK, MODE_STREAM>(grid, block, p, s);
+    if (mode == MODE_FUSED) return go_npt<COLL_BGK, MODE_FUSED>(npt, grid, block, p, s);
+    return go_npt<COLL_BGK, MODE_COLLIDE>(npt, grid, block, p, s);
   }
-  if (mode == MODE_FUSED) return go<COLL_MRT, MODE_FUSED>(grid, block, p, s);
-  if (mode == MODE_COLLIDE) return go<COLL_MRT, MODE_COLLIDE>(grid, block, p, s);
-  return go<COLL_BGK, MODE_STREAM>(grid, block, p, s);  // streaming does not collide
+  if (mode == MODE_FUSED) return go_npt<COLL_MRT, MODE_FUSED>(npt, grid, block, p, s);
+  return go_npt<COLL_MRT, MODE_COLLIDE>(npt, grid, block, p, s);
 }
 
 }  // namespace hlbm

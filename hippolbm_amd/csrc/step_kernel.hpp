@@ -63,6 +63,8 @@ struct StepParams {
   int32_t nprog;
   int32_t prog_planes;  // OR of plane bits used by the program
   int32_t mrt_custom;
+  int32_t gx, gy, gz;   // tile grid of the bulk kernel (tiles of blockDim.x * blockDim.y nodes)
+  int32_t xcd_remap;    // map consecutive dispatch slots to the same XCD's run of tiles
   double cav[2][5];
   double tau;
   double F[3];
@@ -538,19 +540,16 @@ __device__ __forceinline__ int node_planes(int x, int y, int z, int nx, int ny, 
 // Full per-node pipeline.  PROG selects whether the post-stream boundary
 // program is compiled in: the bulk kernel never runs it (its nodes are not on
 // a program plane), the boundary kernel always may.
-template <class S, int COLL, int MODE, bool PROG>
-__device__ __forceinline__ void process_node(const StepParams& p, int x, int y, int z) {
+// Phase 1: the 19 populations arriving at the node (pull + bounce-back), or
+// the node's own populations when the state is pre-collision (MODE_COLLIDE).
+template <class S, int MODE>
+__device__ __forceinline__ void gather_node(const StepParams& p, int x, int y, int z, uint32_t idx, bool wall,
+                                            typename StoreOps<S>::C* f) {
   using O = StoreOps<S>;
   using C = typename O::C;
-  constexpr bool SHIFT = O::shifted;
   const int nx = p.nx, ny = p.ny, nz = p.nz;
-  const uint32_t idx = ((uint32_t)z * (uint32_t)ny + (uint32_t)y) * (uint32_t)nx + (uint32_t)x;
   const S* __restrict__ fin = (const S*)p.fin;
-  S* __restrict__ fout = (S*)p.fout;
   const size_t st = p.stride;
-  const bool wall = p.flags[idx] != 0;
-
-  C f[Q];
   if constexpr (MODE == MODE_COLLIDE) {
 #pragma unroll
     for (int i = 0; i < Q; i++) f[i] = O::ld(fin, (size_t)i * st + idx);
@@ -583,9 +582,21 @@ __device__ __forceinline__ void process_node(const StepParams& p, int x, int y, 
           f[i] = O::ld(fin, (size_t)i * st + sidx);
       }
     });
+  }
+}
 
-    // post-stream program
-    if constexpr (PROG) {
+// Phase 2: the post-stream boundary program (post_bounce_back re-application,
+// lid_driven_cavity, neumann) in deck order, for a node on a program plane.
+template <class S>
+__device__ __forceinline__ void apply_program(const StepParams& p, int x, int y, int z, uint32_t idx, bool wall,
+                                              typename StoreOps<S>::C* f) {
+  using O = StoreOps<S>;
+  using C = typename O::C;
+  constexpr bool SHIFT = O::shifted;
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const S* __restrict__ fin = (const S*)p.fin;
+  const size_t st = p.stride;
+  {
       const int bmask = node_planes(x, y, z, nx, ny, nz);
       {
         for (int k = 0; k < p.nprog; k++) {
@@ -617,8 +628,16 @@ __device__ __forceinline__ void process_node(const StepParams& p, int x, int y, 
         }
       }
     }
-  }
+}
 
+// Phase 3: macro_variables (optionally written out), collision, store.
+template <class S, int COLL, int MODE>
+__device__ __forceinline__ void finish_node(const StepParams& p, uint32_t idx, bool wall, typename StoreOps<S>::C* f) {
+  using O = StoreOps<S>;
+  using C = typename O::C;
+  constexpr bool SHIFT = O::shifted;
+  S* __restrict__ fout = (S*)p.fout;
+  const size_t st = p.stride;
   if constexpr (MODE != MODE_STREAM) {
     if constexpr (!SHIFT) {
       double rho, ux, uy, uz;
@@ -682,16 +701,58 @@ __device__ __forceinline__ void process_node(const StepParams& p, int x, int y, 
   for (int i = 0; i < Q; i++) O::st(fout, (size_t)i * st + idx, f[i]);
 }
 
+// Full per-node pipeline.  PROG selects whether the post-stream boundary
+// program is compiled in: the bulk kernel never runs it (its nodes are not on
+// a program plane), the boundary kernel always may.
+template <class S, int COLL, int MODE, bool PROG>
+__device__ __forceinline__ void process_node(const StepParams& p, int x, int y, int z) {
+  using C = typename StoreOps<S>::C;
+  const uint32_t idx = ((uint32_t)z * (uint32_t)p.ny + (uint32_t)y) * (uint32_t)p.nx + (uint32_t)x;
+  const bool wall = p.flags[idx] != 0;
+  C f[Q];
+  gather_node<S, MODE>(p, x, y, z, idx, wall, f);
+  if constexpr (PROG && MODE != MODE_COLLIDE) apply_program<S>(p, x, y, z, idx, wall, f);
+  finish_node<S, COLL, MODE>(p, idx, wall, f);
+}
 
 // Bulk kernel: every node that is not on a plane of the boundary program.
-template <class S, int COLL, int MODE>
+// A tile is blockDim.x nodes along x by blockDim.y*NPT rows along y; each
+// thread gathers its NPT nodes first (19*NPT loads in flight), then collides
+// and stores them.
+template <class S, int COLL, int MODE, int NPT>
 __global__ void __launch_bounds__(256) step_kernel(const StepParams p) {
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  const int y = blockIdx.y * blockDim.y + threadIdx.y;
-  const int z = blockIdx.z;
-  if (x >= p.nx || y >= p.ny) return;
-  if (MODE != MODE_COLLIDE && (node_planes(x, y, z, p.nx, p.ny, p.nz) & p.prog_planes)) return;
-  process_node<S, COLL, MODE, false>(p, x, y, z);
+  using C = typename StoreOps<S>::C;
+  // 1-D dispatch over tiles.  Workgroups are handed to the 8 XCDs round-robin;
+  // with xcd_remap each XCD walks a contiguous run of tiles, so tiles that
+  // share cache lines (x neighbours, y/z neighbours' flags) share one L2.
+  const uint32_t nt = (uint32_t)p.gx * (uint32_t)p.gy * (uint32_t)p.gz;
+  uint32_t t = blockIdx.x;
+  if (p.xcd_remap) {
+    const uint32_t per = nt / 8u;
+    if (t < per * 8u) t = (t % 8u) * per + t / 8u;
+  }
+  const int bx = (int)(t % (uint32_t)p.gx);
+  const uint32_t r = t / (uint32_t)p.gx;
+  const int by = (int)(r % (uint32_t)p.gy);
+  const int z = (int)(r / (uint32_t)p.gy);
+  const int x = bx * blockDim.x + threadIdx.x;
+  if (x >= p.nx || z >= p.nz) return;
+  C f[NPT][Q];
+  uint32_t idx[NPT];
+  bool wall[NPT], act[NPT];
+#pragma unroll
+  for (int k = 0; k < NPT; k++) {
+    const int y = (by * NPT + k) * blockDim.y + threadIdx.y;
+    act[k] = y < p.ny && !(MODE != MODE_COLLIDE && (node_planes(x, y, z, p.nx, p.ny, p.nz) & p.prog_planes));
+    idx[k] = ((uint32_t)z * (uint32_t)p.ny + (uint32_t)y) * (uint32_t)p.nx + (uint32_t)x;
+    if (act[k]) {
+      wall[k] = p.flags[idx[k]] != 0;
+      gather_node<S, MODE>(p, x, y, z, idx[k], wall[k], f[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NPT; k++)
+    if (act[k]) finish_node<S, COLL, MODE>(p, idx[k], wall[k], f[k]);
 }
 
 // Boundary kernel: the nodes of the program planes, each once (a node on

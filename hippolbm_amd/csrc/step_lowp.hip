@@ -4,35 +4,40 @@
 
 namespace hlbm {
 
-template <class S, int COLL, int MODE>
+template <class S, int COLL, int MODE, int NPT>
 static hipError_t go(dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
-  hipLaunchKernelGGL((step_kernel<S, COLL, MODE>), grid, block, 0, s, p);
+  hipLaunchKernelGGL((step_kernel<S, COLL, MODE, NPT>), grid, block, 0, s, p);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || MODE == MODE_COLLIDE || !p.prog_planes) return e;
   hipLaunchKernelGGL((boundary_kernel<S, COLL, MODE>), dim3(boundary_blocks(p)), dim3(256), 0, s, p);
   return hipGetLastError();
 }
 
-template <class S>
-static hipError_t dispatch(int coll, int mode, dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
-  if (mode == MODE_STREAM) return go<S, COLL_BGK, MODE_STREAM>(grid, block, p, s);
-  if (coll == COLL_BGK) {
-    if (mode == MODE_FUSED) return go<S, COLL_BGK, MODE_FUSED>(grid, block, p, s);
-    return go<S, COLL_BGK, MODE_COLLIDE>(grid, block, p, s);
-  }
-  if (mode == MODE_FUSED) return go<S, COLL_MRT, MODE_FUSED>(grid, block, p, s);
-  return go<S, COLL_MRT, MODE_COLLIDE>(grid, block, p, s);
+template <class S, int COLL, int MODE>
+static hipError_t go_npt(int npt, dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
+  return npt == 2 ? go<S, COLL, MODE, 2>(grid, block, p, s) : go<S, COLL, MODE, 1>(grid, block, p, s);
 }
 
-hipError_t launch_step_lowp(int storage, int coll, int mode, dim3 grid, dim3 block, const StepParams& p,
+template <class S>
+static hipError_t dispatch(int coll, int mode, int npt, dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
+  if (mode == MODE_STREAM) return go_npt<S, COLL_BGK, MODE_STREAM>(npt, grid, block, p, s);
+  if (coll == COLL_BGK) {
+    if (mode == MODE_FUSED) return go_npt<S, COLL_BGK, MODE_FUSED>(npt, grid, block, p, s);
+    return go_npt<S, COLL_BGK, MODE_COLLIDE>(npt, grid, block, p, s);
+  }
+  if (mode == MODE_FUSED) return go_npt<S, COLL_MRT, MODE_FUSED>(npt, grid, block, p, s);
+  return go_npt<S, COLL_MRT, MODE_COLLIDE>(npt, grid, block, p, s);
+}
+
+hipError_t launch_step_lowp(int storage, int coll, int mode, int npt, dim3 grid, dim3 block, const StepParams& p,
                             hipStream_t s) {
   switch (storage) {
     case 1:
-      return dispatch<float>(coll, mode, grid, block, p, s);
+      return dispatch<float>(coll, mode, npt, grid, block, p, s);
     case 2:
-      return dispatch<_Float16>(coll, mode, grid, block, p, s);
+      return dispatch<_Float16>(coll, mode, npt, grid, block, p, s);
     case 3:
-      return dispatch<bf16s>(coll, mode, grid, block, p, s);
+      return dispatch<bf16s>(coll, mode, npt, grid, block, p, s);
     default:
       return hipErrorInvalidValue;
   }
