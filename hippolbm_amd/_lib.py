@@ -33,6 +33,8 @@ SYMBOLS = (
     "hlbm_set_flags", "hlbm_get_flags", "hlbm_set_distribution", "hlbm_set_populations_f64",
     "hlbm_get_populations_f64", "hlbm_step", "hlbm_timestep", "hlbm_synchronize", "hlbm_macro", "hlbm_get_macro",
     "hlbm_reduce_stats", "hlbm_ghost_update_ref", "hlbm_copy_bench",
+    "hlbm_batch_create", "hlbm_batch_destroy", "hlbm_batch_submit", "hlbm_batch_run", "hlbm_batch_status",
+    "hlbm_batch_times", "hlbm_batch_result", "hlbm_batch_info_get", "hlbm_clock",
 )
 
 

@@ -14,19 +14,13 @@
 #include <vector>
 
 #include "../../include/hlbm.h"
+#include "common.hpp"
 #include "launch.hpp"
 #include "util_kernels.hpp"
 
 using namespace hlbm;
 
 namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
 
 #define HIP_TRY(expr)                                                                        \
   do {                                                                                       \
@@ -128,7 +122,7 @@ static int materialize(hlbm_domain* h, hipStream_t s) {
 
 extern "C" {
 
-const char* hlbm_last_error(void) { return g_err.c_str(); }
+const char* hlbm_last_error(void) { return last_error().c_str(); }
 
 const char* hlbm_version(void) { return "hippolbm-mi355x 0.1 (gfx950)"; }
 

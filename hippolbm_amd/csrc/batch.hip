@@ -1,0 +1,557 @@
+// batch.hip -- batched domains and the solve server (hlbm_batch_* in include/hlbm.h).
+//
+// Host code.  Solves live in slices of two population arenas (A/B) and one
+// flag arena, carved by a first-fit allocator against an HBM budget.  A chunk
+// is `chunk_steps` launches of batch_kernel (one per time step, captured once
+// as a HIP graph), each walking the tile table of all running solves.
+// Admission runs the collide-only kernel on the new slice; retirement runs the
+// stream-only kernel, macro_variables, and copies the result out.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/hlbm.h"
+#include "common.hpp"
+#include "launch.hpp"
+#include "util_kernels.hpp"
+
+using namespace hlbm;
+
+namespace {
+
+int bfail(int code, const std::string& m) { return fail(code, m); }
+
+double now_s() {
+  using namespace std::chrono;
+  static const steady_clock::time_point t0 = steady_clock::now();
+  return duration<double>(steady_clock::now() - t0).count();
+}
+
+// first-fit allocator over [0, size) in abstract units
+struct FreeList {
+  std::map<uint64_t, uint64_t> free;  // offset -> length
+  uint64_t size = 0, used = 0;
+  void init(uint64_t n) {
+    size = n;
+    used = 0;
+    free.clear();
+    if (n) free[0] = n;
+  }
+  bool alloc(uint64_t n, uint64_t* off) {
+    for (auto it = free.begin(); it != free.end(); ++it) {
+      if (it->second >= n) {
+        *off = it->first;
+        const uint64_t rest = it->second - n, roff = it->first + n;
+        free.erase(it);
+        if (rest) free[roff] = rest;
+        used += n;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(uint64_t off, uint64_t n) {
+    used -= n;
+    auto it = free.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free.erase(nx);
+    }
+    if (it != free.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free.erase(it);
+      }
+    }
+  }
+};
+
+struct Solve {
+  int64_t id = 0;
+  hlbm_solve_desc desc{};
+  std::vector<hlbm_obstacle> obs;
+  int state = HLBM_SOLVE_QUEUED;
+  int64_t steps_done = 0;
+  int fused_left = 0;
+  int parity = 0;
+  uint32_t N = 0;
+  uint64_t stride = 0, f_off = 0, f_len = 0, flag_off = 0, flag_len = 0;
+  BatchDomain dom{};
+  double t_submit = 0, t_admit = 0, t_done = 0;
+  std::vector<double> rho, u, f;
+};
+
+uint64_t round_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace
+
+struct hlbm_batch {
+  hlbm_batch_desc d{};
+  size_t esize = 8;
+  uint64_t arena_elems = 0, flag_bytes = 0;
+  void* arena[2] = {nullptr, nullptr};
+  uint8_t* flags = nullptr;
+  FreeList fl_pop, fl_flag;
+  int64_t next_id = 1;
+  std::map<int64_t, Solve> solves;
+  std::deque<int64_t> queue;
+  std::vector<int64_t> running;
+  BatchDomain* d_doms = nullptr;
+  BatchTile* d_tiles = nullptr;
+  uint32_t* d_ntiles = nullptr;
+  BatchDomain* h_doms = nullptr;  // pinned staging
+  BatchTile* h_tiles = nullptr;
+  uint32_t* h_ntiles = nullptr;
+  uint64_t max_doms = 0, max_tiles = 0;
+  double* m_tmp = nullptr;  // rho + u of one retiring solve (4 * max_nodes doubles)
+  double* f_tmp = nullptr;  // one FP64 population plane
+  uint64_t max_nodes = 0;
+  unsigned blocks = 2048;
+  hipStream_t stream = nullptr;
+  hipGraphExec_t graph = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool tiles_dirty = true;
+  int64_t chunks = 0, n_done = 0;
+  double node_updates = 0, gpu_ms = 0;
+};
+
+static StepParams single_params(hlbm_batch* b, const Solve& s, int parity) {
+  StepParams p{};
+  p.nx = s.dom.nx;
+  p.ny = s.dom.ny;
+  p.nz = s.dom.nz;
+  for (int k = 0; k < 3; k++) p.per[k] = s.dom.per[k];
+  p.N = s.N;
+  p.stride = s.stride;
+  p.fin = (const char*)b->arena[parity] + s.f_off * b->esize;
+  p.fout = (char*)b->arena[parity ^ 1] + s.f_off * b->esize;
+  p.flags = b->flags + s.flag_off;
+  p.wall_bb = s.dom.wall_bb;
+  p.plane_bb = 1;
+  p.cav_on[0] = s.dom.cav_on[0];
+  p.cav_on[1] = s.dom.cav_on[1];
+  p.cav_gather = 1;
+  p.mrt_custom = s.dom.mrt_custom;
+  for (int k = 0; k < 5; k++) {
+    p.cav[0][k] = s.dom.cav[0][k];
+    p.cav[1][k] = s.dom.cav[1][k];
+  }
+  p.tau = s.dom.tau;
+  for (int i = 0; i < Q; i++) p.ef[i] = s.dom.ef[i];
+  for (int k = 0; k < 10; k++) p.mrt[k] = s.dom.mrt[k];
+  p.gx = (s.dom.nx + 127) / 128;
+  p.gy = (s.dom.ny + 1) / 2;
+  p.gz = s.dom.nz;
+  p.xcd_remap = 1;
+  return p;
+}
+
+static hipError_t single_launch(hlbm_batch* b, const Solve& s, int mode, int parity) {
+  StepParams p = single_params(b, s, parity);
+  const dim3 block(128, 2, 1), grid((unsigned)p.gx * p.gy * p.gz, 1, 1);
+  const int coll = s.dom.coll;
+  if (b->d.storage == HLBM_F64) return launch_step_f64(coll, mode, 1, grid, block, p, b->stream);
+  return launch_step_lowp(b->d.storage, coll, mode, 1, grid, block, p, b->stream);
+}
+
+static hipError_t chunk_launch(hlbm_batch* b, int step) {
+  BatchParams bp;
+  bp.doms = b->d_doms;
+  bp.tiles = b->d_tiles;
+  bp.ntiles = b->d_ntiles;
+  bp.arena[0] = b->arena[0];
+  bp.arena[1] = b->arena[1];
+  bp.flags = b->flags;
+  bp.step = step;
+  if (b->d.storage == HLBM_F64) return launch_batch_f64(b->blocks, bp, b->stream);
+  return launch_batch_lowp(b->d.storage, b->blocks, bp, b->stream);
+}
+
+static int build_graph(hlbm_batch* b) {
+  hipGraph_t g = nullptr;
+  if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return bfail(HLBM_ERR_HIP, "graph capture begin failed");
+  for (int k = 0; k < b->d.chunk_steps; k++) {
+    if (chunk_launch(b, k) != hipSuccess) {
+      (void)hipStreamEndCapture(b->stream, &g);
+      if (g) (void)hipGraphDestroy(g);
+      return bfail(HLBM_ERR_HIP, "batch kernel launch failed during capture");
+    }
+  }
+  if (hipStreamEndCapture(b->stream, &g) != hipSuccess) return bfail(HLBM_ERR_HIP, "graph capture end failed");
+  hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return bfail(HLBM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(e));
+  return HLBM_OK;
+}
+
+extern "C" {
+
+double hlbm_clock(void) { return now_s(); }
+
+int hlbm_batch_create(const hlbm_batch_desc* desc, hlbm_batch** out) {
+  if (!desc || !out) return bfail(HLBM_ERR_ARG, "NULL argument");
+  *out = nullptr;
+  if (desc->storage < HLBM_F64 || desc->storage > HLBM_BF16) return bfail(HLBM_ERR_ARG, "bad storage");
+  if (desc->chunk_steps < 1 || desc->chunk_steps > 100000) return bfail(HLBM_ERR_ARG, "bad chunk_steps");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return bfail(HLBM_ERR_NODEVICE, "no HIP device visible");
+  hlbm_batch* b = new hlbm_batch();
+  b->d = *desc;
+  b->esize = desc->storage == HLBM_F64 ? 8 : (desc->storage == HLBM_F32 ? 4 : 2);
+  // budget = 2 arenas of 19 planes + flags: per node 38*esize + 1 bytes
+  const uint64_t per_node = 38 * b->esize + 1;
+  b->max_nodes = desc->budget_bytes / per_node;
+  if (b->max_nodes < 4096) {
+    delete b;
+    return bfail(HLBM_ERR_ARG, "budget too small");
+  }
+  b->arena_elems = b->max_nodes * 19;
+  b->flag_bytes = b->max_nodes + 4096 * 256;
+  b->max_doms = 16384;
+  b->max_tiles = b->max_nodes / 256 + b->max_doms;
+  hipDeviceProp_t prop;
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+    b->blocks = (unsigned)prop.multiProcessorCount * 8u;
+  auto cleanup = [&](const char* what) {
+    hlbm_batch_destroy(b);
+    return bfail(HLBM_ERR_NOMEM, std::string("batch allocation failed: ") + what);
+  };
+  if (hipMalloc(&b->arena[0], b->arena_elems * b->esize) != hipSuccess) return cleanup("arena A");
+  if (hipMalloc(&b->arena[1], b->arena_elems * b->esize) != hipSuccess) return cleanup("arena B");
+  if (hipMalloc((void**)&b->flags, b->flag_bytes) != hipSuccess) return cleanup("flags");
+  if (hipMalloc((void**)&b->d_doms, b->max_doms * sizeof(BatchDomain)) != hipSuccess) return cleanup("domains");
+  if (hipMalloc((void**)&b->d_tiles, b->max_tiles * sizeof(BatchTile)) != hipSuccess) return cleanup("tiles");
+  if (hipMalloc((void**)&b->d_ntiles, sizeof(uint32_t)) != hipSuccess) return cleanup("tile count");
+  if (hipHostMalloc((void**)&b->h_doms, b->max_doms * sizeof(BatchDomain)) != hipSuccess) return cleanup("pinned");
+  if (hipHostMalloc((void**)&b->h_tiles, b->max_tiles * sizeof(BatchTile)) != hipSuccess) return cleanup("pinned");
+  if (hipHostMalloc((void**)&b->h_ntiles, sizeof(uint32_t)) != hipSuccess) return cleanup("pinned");
+  if (hipStreamCreate(&b->stream) != hipSuccess) return cleanup("stream");
+  if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) return cleanup("events");
+  *b->h_ntiles = 0;
+  if (hipMemcpy(b->d_ntiles, b->h_ntiles, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup("tile count init");
+  b->fl_pop.init(b->arena_elems);
+  b->fl_flag.init(b->flag_bytes);
+  if (desc->use_graph) {
+    int rc = build_graph(b);
+    if (rc) {
+      std::string m = last_error();
+      hlbm_batch_destroy(b);
+      return bfail(rc, m);
+    }
+  }
+  *out = b;
+  return HLBM_OK;
+}
+
+int hlbm_batch_destroy(hlbm_batch* b) {
+  if (!b) return HLBM_OK;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  for (void* p : {b->arena[0], b->arena[1], (void*)b->flags, (void*)b->d_doms, (void*)b->d_tiles, (void*)b->d_ntiles,
+                  (void*)b->m_tmp, (void*)b->f_tmp})
+    if (p) (void)hipFree(p);
+  for (void* p : {(void*)b->h_doms, (void*)b->h_tiles, (void*)b->h_ntiles})
+    if (p) (void)hipHostFree(p);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return HLBM_OK;
+}
+
+int hlbm_batch_submit(hlbm_batch* b, const hlbm_solve_desc* s, int64_t* id) {
+  if (!b || !s || !id) return bfail(HLBM_ERR_ARG, "NULL argument");
+  for (int k = 0; k < 3; k++)
+    if (s->nodes[k] < 2) return bfail(HLBM_ERR_ARG, "nodes must be >= 2");
+  if (s->steps < 0 || !(s->tau > 0.0)) return bfail(HLBM_ERR_ARG, "bad steps / tau");
+  if (s->collision != HLBM_BGK && s->collision != HLBM_MRT) return bfail(HLBM_ERR_ARG, "bad collision");
+  if ((s->cavity_on[0] || s->cavity_on[1]) && s->periodic[2])
+    return bfail(HLBM_ERR_ARG, "a moving wall needs a closed z axis");
+  if (s->n_obstacles > 0 && (!s->obstacles || !(s->dx > 0.0))) return bfail(HLBM_ERR_ARG, "obstacles need dx > 0");
+  const uint64_t N = (uint64_t)s->nodes[0] * s->nodes[1] * s->nodes[2];
+  const uint64_t stride = round_up(N, 128);
+  if (stride * 19 > b->arena_elems || N > b->max_nodes) return bfail(HLBM_ERR_NOMEM, "solve can never fit the budget");
+  Solve v;
+  v.id = b->next_id++;
+  v.desc = *s;
+  v.desc.obstacles = nullptr;
+  if (s->n_obstacles > 0) v.obs.assign(s->obstacles, s->obstacles + s->n_obstacles);
+  v.N = (uint32_t)N;
+  v.stride = stride;
+  v.f_len = stride * 19;
+  v.flag_len = round_up(N, 256);
+  v.t_submit = now_s();
+  BatchDomain& d = v.dom;
+  d.nx = s->nodes[0];
+  d.ny = s->nodes[1];
+  d.nz = s->nodes[2];
+  for (int k = 0; k < 3; k++) d.per[k] = s->periodic[k] ? 1 : 0;
+  d.coll = s->collision == HLBM_MRT ? COLL_MRT : COLL_BGK;
+  d.wall_bb = s->wall_bb && s->n_obstacles > 0 ? 1 : 0;
+  d.mrt_custom = s->mrt_custom;
+  d.tau = s->tau;
+  for (int i = 0; i < Q; i++) d.ef[i] = CX(i) * s->fext[0] + CY(i) * s->fext[1] + CZ(i) * s->fext[2];
+  for (int k = 0; k < 10; k++) d.mrt[k] = s->mrt_rates[k];
+  const double L = d.nz;  // cavity.hpp:58-63
+  for (int side = 0; side < 2; side++) {
+    d.cav_on[side] = s->cavity_on[side] ? 1 : 0;
+    const double uxx = s->cavity_u[side][0] * (1 + 0.5 / (L - 1));
+    const double uyy = s->cavity_u[side][1] * (1 + 0.5 / (L - 1));
+    const double uzz = s->cavity_u[side][2] * (1 + 0.5 / (L - 1));
+    for (int j = 0; j < 5; j++) {
+      const int q = Faces::in[4 + side][j];
+      d.cav[side][j] = 6. * WD(q) * (CX(q) * uxx + CY(q) * uyy + CZ(q) * uzz);
+    }
+  }
+  *id = v.id;
+  b->solves.emplace(v.id, std::move(v));
+  b->queue.push_back(*id);
+  return HLBM_OK;
+}
+
+static int admit_one(hlbm_batch* b, Solve& s) {
+  hipStream_t st = b->stream;
+  if (hipMemsetAsync(b->flags + s.flag_off, 0, s.flag_len, st) != hipSuccess) return bfail(HLBM_ERR_HIP, "memset");
+  if (!s.obs.empty()) {
+    std::vector<ObstacleDev> od;
+    const int nn[3] = {s.dom.nx, s.dom.ny, s.dom.nz};
+    const double dx = s.desc.dx;
+    for (const hlbm_obstacle& o : s.obs) {
+      ObstacleDev v{};
+      v.kind = o.kind;
+      bool inside = true;
+      for (int k = 0; k < 3; k++) {
+        const double lo = o.kind == 0 ? o.p0[k] - o.radius : o.p0[k];
+        const double hi = o.kind == 0 ? o.p0[k] + o.radius : o.p1[k];
+        const int a = int(lo / dx), c = int(hi / dx);
+        if (c < 0 || a > nn[k] - 1) inside = false;
+        v.lo[k] = std::max(a, 0);
+        v.hi[k] = std::min(c, nn[k] - 1);
+        v.p0[k] = o.p0[k];
+        v.p1[k] = o.p1[k];
+      }
+      v.r2 = o.radius * o.radius;
+      if (inside) od.push_back(v);
+    }
+    if (!od.empty()) {
+      ObstacleDev* dob = nullptr;
+      if (hipMalloc(&dob, od.size() * sizeof(ObstacleDev)) != hipSuccess) return bfail(HLBM_ERR_NOMEM, "obstacles");
+      hipError_t e = hipMemcpyAsync(dob, od.data(), od.size() * sizeof(ObstacleDev), hipMemcpyHostToDevice, st);
+      if (e == hipSuccess) e = launch_rasterize(b->flags + s.flag_off, s.dom.nx, s.dom.ny, s.dom.nz, dx, dob, (int)od.size(), st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      (void)hipFree(dob);
+      if (e != hipSuccess) return bfail(HLBM_ERR_HIP, "rasterize");
+    }
+  }
+  int lo[3] = {0, 0, 0}, hi[3] = {s.dom.nx - 1, s.dom.ny - 1, s.dom.nz - 1};
+  void* fa = (char*)b->arena[0] + s.f_off * b->esize;
+  if (launch_set_distribution(b->d.storage, fa, s.stride, s.dom.nx, s.dom.ny, lo, hi, s.desc.init_rho, st) != hipSuccess)
+    return bfail(HLBM_ERR_HIP, "set_distribution");
+  s.parity = 0;
+  if (s.desc.steps > 0) {
+    if (single_launch(b, s, MODE_COLLIDE, 0) != hipSuccess) return bfail(HLBM_ERR_HIP, "collide");
+    s.parity = 1;
+    s.steps_done = 1;
+    s.fused_left = s.desc.steps - 1;
+  } else {
+    s.fused_left = 0;
+  }
+  s.state = HLBM_SOLVE_RUNNING;
+  s.t_admit = now_s();
+  b->node_updates += s.desc.steps > 0 ? (double)s.N : 0.0;
+  return HLBM_OK;
+}
+
+static int retire_one(hlbm_batch* b, Solve& s) {
+  hipStream_t st = b->stream;
+  int par = s.parity;
+  if (s.desc.steps > 0) {
+    if (single_launch(b, s, MODE_STREAM, par) != hipSuccess) return bfail(HLBM_ERR_HIP, "stream");
+    par ^= 1;
+  }
+  if (!b->m_tmp && hipMalloc((void**)&b->m_tmp, b->max_nodes * 4 * 8) != hipSuccess) return bfail(HLBM_ERR_NOMEM, "tmp");
+  const void* fcur = (const char*)b->arena[par] + s.f_off * b->esize;
+  // walls keep rho = 0 (macro_variables never writes them)
+  if (hipMemsetAsync(b->m_tmp, 0, (size_t)s.N * 8, st) != hipSuccess) return bfail(HLBM_ERR_HIP, "memset");
+  if (launch_macro(b->d.storage, fcur, s.stride, s.N, b->flags + s.flag_off, b->m_tmp, b->m_tmp + s.N, st) != hipSuccess)
+    return bfail(HLBM_ERR_HIP, "macro");
+  s.rho.resize(s.N);
+  s.u.resize(3 * (size_t)s.N);
+  if (hipMemcpyAsync(s.rho.data(), b->m_tmp, s.N * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
+      hipMemcpyAsync(s.u.data(), b->m_tmp + s.N, 3 * (size_t)s.N * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
+    return bfail(HLBM_ERR_HIP, "copy out");
+  if (s.desc.keep_populations) {
+    if (!b->f_tmp && hipMalloc((void**)&b->f_tmp, b->max_nodes * 8) != hipSuccess) return bfail(HLBM_ERR_NOMEM, "tmp");
+    s.f.resize((size_t)19 * s.N);
+    for (int i = 0; i < Q; i++) {
+      const void* plane = (const char*)fcur + (size_t)i * s.stride * b->esize;
+      if (launch_convert(b->d.storage, false, (void*)plane, b->f_tmp, s.N, i, st) != hipSuccess ||
+          hipMemcpyAsync(s.f.data() + (size_t)i * s.N, b->f_tmp, s.N * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
+        return bfail(HLBM_ERR_HIP, "populations out");
+    }
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) return bfail(HLBM_ERR_HIP, "sync");
+  b->fl_pop.release(s.f_off, s.f_len);
+  b->fl_flag.release(s.flag_off, s.flag_len);
+  s.state = HLBM_SOLVE_DONE;
+  s.t_done = now_s();
+  b->n_done++;
+  return HLBM_OK;
+}
+
+int hlbm_batch_run(hlbm_batch* b, int32_t* n_finished) {
+  if (!b) return bfail(HLBM_ERR_ARG, "NULL batch");
+  int finished = 0;
+  // admission (first fit, in arrival order, backfilling smaller solves)
+  for (auto it = b->queue.begin(); it != b->queue.end();) {
+    Solve& s = b->solves[*it];
+    if (b->running.size() >= b->max_doms) break;
+    uint64_t foff, goff;
+    if (!b->fl_pop.alloc(s.f_len, &foff)) {
+      ++it;
+      continue;
+    }
+    if (!b->fl_flag.alloc(s.flag_len, &goff)) {
+      b->fl_pop.release(foff, s.f_len);
+      ++it;
+      continue;
+    }
+    s.f_off = foff;
+    s.flag_off = goff;
+    s.dom.f_off = foff;
+    s.dom.stride = s.stride;
+    s.dom.flag_off = goff;
+    int rc = admit_one(b, s);
+    if (rc) return rc;
+    b->running.push_back(s.id);
+    it = b->queue.erase(it);
+    b->tiles_dirty = true;
+  }
+  // per-domain chunk lengths and the tile table
+  const int K = b->d.chunk_steps;
+  uint32_t nt = 0;
+  for (size_t k = 0; k < b->running.size(); k++) {
+    Solve& s = b->solves[b->running[k]];
+    s.dom.active_steps = std::min(K, s.fused_left);
+    s.dom.parity = s.parity;
+    b->h_doms[k] = s.dom;
+    if (b->tiles_dirty) {
+      for (uint32_t first = 0; first < s.N; first += 256) b->h_tiles[nt++] = BatchTile{(uint32_t)k, first};
+    }
+  }
+  hipStream_t st = b->stream;
+  if (!b->running.empty()) {
+    if (hipMemcpyAsync(b->d_doms, b->h_doms, b->running.size() * sizeof(BatchDomain), hipMemcpyHostToDevice, st) != hipSuccess)
+      return bfail(HLBM_ERR_HIP, "upload domains");
+    if (b->tiles_dirty) {
+      *b->h_ntiles = nt;
+      if (hipMemcpyAsync(b->d_tiles, b->h_tiles, (size_t)nt * sizeof(BatchTile), hipMemcpyHostToDevice, st) != hipSuccess ||
+          hipMemcpyAsync(b->d_ntiles, b->h_ntiles, sizeof(uint32_t), hipMemcpyHostToDevice, st) != hipSuccess)
+        return bfail(HLBM_ERR_HIP, "upload tiles");
+      b->tiles_dirty = false;
+    }
+    int steps_needed = 0;
+    for (int64_t id : b->running) steps_needed = std::max(steps_needed, b->solves[id].dom.active_steps);
+    if (steps_needed > 0) {
+      if (hipEventRecord(b->ev0, st) != hipSuccess) return bfail(HLBM_ERR_HIP, "event");
+      if (b->graph && steps_needed == K) {
+        if (hipGraphLaunch(b->graph, st) != hipSuccess) return bfail(HLBM_ERR_HIP, "graph launch");
+      } else {
+        for (int k = 0; k < steps_needed; k++)
+          if (chunk_launch(b, k) != hipSuccess) return bfail(HLBM_ERR_HIP, "batch kernel launch");
+      }
+      if (hipEventRecord(b->ev1, st) != hipSuccess || hipEventSynchronize(b->ev1) != hipSuccess)
+        return bfail(HLBM_ERR_HIP, "chunk sync");
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
+      b->gpu_ms += ms;
+      b->chunks++;
+    }
+    for (int64_t id : b->running) {
+      Solve& s = b->solves[id];
+      const int a = s.dom.active_steps;
+      s.parity ^= (a & 1);
+      s.fused_left -= a;
+      s.steps_done += a;
+      b->node_updates += (double)a * s.N;
+    }
+  }
+  // retirement
+  std::vector<int64_t> keep;
+  for (int64_t id : b->running) {
+    Solve& s = b->solves[id];
+    if (s.fused_left == 0) {
+      int rc = retire_one(b, s);
+      if (rc) return rc;
+      finished++;
+      b->tiles_dirty = true;
+    } else {
+      keep.push_back(id);
+    }
+  }
+  b->running.swap(keep);
+  if (n_finished) *n_finished = finished;
+  return HLBM_OK;
+}
+
+int hlbm_batch_status(hlbm_batch* b, int64_t id, int32_t* state, int64_t* steps_done) {
+  if (!b) return bfail(HLBM_ERR_ARG, "NULL batch");
+  auto it = b->solves.find(id);
+  if (it == b->solves.end()) return bfail(HLBM_ERR_ARG, "unknown solve id");
+  if (state) *state = it->second.state;
+  if (steps_done) *steps_done = it->second.steps_done;
+  return HLBM_OK;
+}
+
+int hlbm_batch_times(hlbm_batch* b, int64_t id, double* t_submit, double* t_admit, double* t_done) {
+  if (!b) return bfail(HLBM_ERR_ARG, "NULL batch");
+  auto it = b->solves.find(id);
+  if (it == b->solves.end()) return bfail(HLBM_ERR_ARG, "unknown solve id");
+  if (t_submit) *t_submit = it->second.t_submit;
+  if (t_admit) *t_admit = it->second.t_admit;
+  if (t_done) *t_done = it->second.t_done;
+  return HLBM_OK;
+}
+
+int hlbm_batch_result(hlbm_batch* b, int64_t id, double* rho, double* u, double* f) {
+  if (!b) return bfail(HLBM_ERR_ARG, "NULL batch");
+  auto it = b->solves.find(id);
+  if (it == b->solves.end()) return bfail(HLBM_ERR_ARG, "unknown solve id");
+  Solve& s = it->second;
+  if (s.state != HLBM_SOLVE_DONE) return bfail(HLBM_ERR_ARG, "solve not finished");
+  if (rho) std::memcpy(rho, s.rho.data(), s.rho.size() * 8);
+  if (u) std::memcpy(u, s.u.data(), s.u.size() * 8);
+  if (f) {
+    if (!s.desc.keep_populations) return bfail(HLBM_ERR_ARG, "populations were not kept (keep_populations = 0)");
+    std::memcpy(f, s.f.data(), s.f.size() * 8);
+  }
+  b->solves.erase(it);
+  return HLBM_OK;
+}
+
+int hlbm_batch_info_get(hlbm_batch* b, hlbm_batch_info* out) {
+  if (!b || !out) return bfail(HLBM_ERR_ARG, "NULL argument");
+  out->queued = (int64_t)b->queue.size();
+  out->running = (int64_t)b->running.size();
+  out->done = b->n_done;
+  out->chunks = b->chunks;
+  out->bytes_in_use = b->fl_pop.used * b->esize * 2 + b->fl_flag.used;
+  out->budget_bytes = b->d.budget_bytes;
+  out->node_updates = b->node_updates;
+  out->gpu_ms = b->gpu_ms;
+  return HLBM_OK;
+}
+
+}  // extern "C"

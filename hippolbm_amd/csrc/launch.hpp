@@ -25,4 +25,8 @@ hipError_t launch_step_f64(int coll, int mode, int npt, dim3 grid, dim3 block, c
 hipError_t launch_step_lowp(int storage, int coll, int mode, int npt, dim3 grid, dim3 block, const StepParams& p,
                             hipStream_t s);
 
+// batched domains, one persistent launch (blocks = resident workgroups)
+hipError_t launch_batch_f64(unsigned blocks, const BatchParams& bp, hipStream_t s);
+hipError_t launch_batch_lowp(int storage, unsigned blocks, const BatchParams& bp, hipStream_t s);
+
 }  // namespace hlbm

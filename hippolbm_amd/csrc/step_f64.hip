@@ -29,4 +29,9 @@ hipError_t launch_step_f64(int coll, int mode, int npt, dim3 grid, dim3 block, c
   return go_npt<COLL_MRT, MODE_COLLIDE>(npt, grid, block, p, s);
 }
 
+hipError_t launch_batch_f64(unsigned blocks, const BatchParams& bp, hipStream_t s) {
+  hipLaunchKernelGGL((batch_kernel<double>), dim3(blocks), dim3(256), 0, s, bp);
+  return hipGetLastError();
+}
+
 }  // namespace hlbm

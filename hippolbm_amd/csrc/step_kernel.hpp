@@ -634,7 +634,6 @@ __device__ __forceinline__ void apply_program(const StepParams& p, int x, int y,
 template <class S, int COLL, int MODE>
 __device__ __forceinline__ void finish_node(const StepParams& p, uint32_t idx, bool wall, typename StoreOps<S>::C* f) {
   using O = StoreOps<S>;
-  using C = typename O::C;
   constexpr bool SHIFT = O::shifted;
   S* __restrict__ fout = (S*)p.fout;
   const size_t st = p.stride;
@@ -778,6 +777,114 @@ __global__ void __launch_bounds__(256) boundary_kernel(const StepParams p) {
       return;
     }
     t -= cnt;
+  }
+}
+
+}  // namespace hlbm
+
+namespace hlbm {
+
+// ---------------------------------------------------------------------------
+// Batched domains: many independent subdomains of uneven sizes advance in one
+// persistent launch per step.  Each domain owns a slice of two population
+// arenas (A/B) and of a flag arena; a tile table lists (domain, first node)
+// pairs of 256 consecutive nodes.  Per-domain parameters replace the uniform
+// kernel arguments of the single-domain path; the per-node pipeline is the
+// same code (gather_node / finish_node), so a domain's populations are the
+// same alone and in any mix.
+// ---------------------------------------------------------------------------
+struct BatchDomain {
+  int32_t nx, ny, nz;
+  int32_t per[3];
+  int32_t coll;          // COLL_BGK / COLL_MRT
+  int32_t wall_bb;
+  int32_t cav_on[2];
+  int32_t active_steps;  // steps this domain takes in the current chunk
+  int32_t parity;        // arena holding f* at the start of the chunk
+  int32_t mrt_custom;
+  int32_t pad;
+  uint64_t f_off;        // element offset of population plane 0 in an arena
+  uint64_t stride;       // elements between population planes
+  uint64_t flag_off;     // byte offset in the flag arena
+  double tau;
+  double ef[Q];
+  double cav[2][5];
+  double mrt[10];
+};
+
+struct BatchTile {
+  uint32_t dom;
+  uint32_t first;  // first node (linear index) of the tile
+};
+
+struct BatchParams {
+  const BatchDomain* doms;
+  const BatchTile* tiles;
+  const uint32_t* ntiles;  // device-resident count (graphs stay valid as the mix changes)
+  void* arena[2];
+  const uint8_t* flags;
+  int32_t step;            // step index inside the chunk
+};
+
+template <class S>
+__device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, const BatchDomain& d, int parity) {
+  StepParams p;
+  p.nx = d.nx;
+  p.ny = d.ny;
+  p.nz = d.nz;
+  p.per[0] = d.per[0];
+  p.per[1] = d.per[1];
+  p.per[2] = d.per[2];
+  p.N = (uint32_t)d.nx * (uint32_t)d.ny * (uint32_t)d.nz;
+  p.stride = d.stride;
+  p.fin = (const void*)((const S*)bp.arena[parity] + d.f_off);
+  p.fout = (void*)((S*)bp.arena[parity ^ 1] + d.f_off);
+  p.flags = bp.flags + d.flag_off;
+  p.m0 = nullptr;
+  p.m1 = nullptr;
+  p.write_macro = 0;
+  p.wall_bb = d.wall_bb;
+  p.plane_bb = 1;
+  p.cav_on[0] = d.cav_on[0];
+  p.cav_on[1] = d.cav_on[1];
+  p.cav_gather = 1;
+  p.nprog = 0;
+  p.prog_planes = 0;
+  p.mrt_custom = d.mrt_custom;
+  for (int k = 0; k < 5; k++) {
+    p.cav[0][k] = d.cav[0][k];
+    p.cav[1][k] = d.cav[1][k];
+  }
+  p.tau = d.tau;
+  for (int i = 0; i < Q; i++) p.ef[i] = d.ef[i];
+  for (int k = 0; k < 10; k++) p.mrt[k] = d.mrt[k];
+  return p;
+}
+
+template <class S>
+__global__ void __launch_bounds__(256) batch_kernel(const BatchParams bp) {
+  using C = typename StoreOps<S>::C;
+  const uint32_t nt = *bp.ntiles;
+  for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+    const BatchTile tl = bp.tiles[t];
+    const BatchDomain& d = bp.doms[tl.dom];
+    if (bp.step >= d.active_steps) continue;
+    const int parity = (d.parity + bp.step) & 1;
+    const StepParams p = batch_step_params<S>(bp, d, parity);
+    const uint32_t n = tl.first + threadIdx.x;
+    if (n >= p.N) continue;
+    const uint32_t nxy = (uint32_t)p.nx * (uint32_t)p.ny;
+    const int z = (int)(n / nxy);
+    const uint32_t rem = n - (uint32_t)z * nxy;
+    const int y = (int)(rem / (uint32_t)p.nx);
+    const int x = (int)(rem - (uint32_t)y * (uint32_t)p.nx);
+    const bool wall = p.flags[n] != 0;
+    C f[Q];
+    gather_node<S, MODE_FUSED>(p, x, y, z, n, wall, f);
+    if (d.coll == COLL_MRT)
+      finish_node<S, COLL_MRT, MODE_FUSED>(p, n, wall, f);
+    else
+      finish_node<S, COLL_BGK, MODE_FUSED>(p, n, wall, f);
   }
 }
 

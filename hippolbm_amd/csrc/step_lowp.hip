@@ -43,4 +43,21 @@ hipError_t launch_step_lowp(int storage, int coll, int mode, int npt, dim3 grid,
   }
 }
 
+hipError_t launch_batch_lowp(int storage, unsigned blocks, const BatchParams& bp, hipStream_t s) {
+  switch (storage) {
+    case 1:
+      hipLaunchKernelGGL((batch_kernel<float>), dim3(blocks), dim3(256), 0, s, bp);
+      break;
+    case 2:
+      hipLaunchKernelGGL((batch_kernel<_Float16>), dim3(blocks), dim3(256), 0, s, bp);
+      break;
+    case 3:
+      hipLaunchKernelGGL((batch_kernel<bf16s>), dim3(blocks), dim3(256), 0, s, bp);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 }  // namespace hlbm

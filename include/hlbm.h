@@ -174,6 +174,73 @@ int hlbm_reduce_stats(hlbm_domain* h, double out[7]);
  * deck that calls update_ghost explicitly keeps working. */
 int hlbm_ghost_update_ref(const hlbm_domain* h, double* f_ref);
 
+/* ---- batched domains and the solve server -------------------------------
+ * Many independent domains of uneven sizes advance together: one persistent
+ * kernel launch per time step walks a tile table covering every running
+ * domain, with per-domain tau / force / boundary tables.  Chunks of
+ * `chunk_steps` steps are captured once as a HIP graph and replayed.
+ * Requests ("solves") are submitted at any time; each hlbm_batch_run() call
+ * admits queued solves whose populations fit the HBM budget, runs one chunk,
+ * and retires finished solves (their rho/u are copied to the host and their
+ * device memory is released).  This is the serving counterpart of running
+ * hippoLBM decks one after another; it has no operator of its own in the
+ * reference.  Boundary conditions per solve: periodic or closed faces
+ * (pre_bounce_back + post_bounce_back on every closed face), optional moving
+ * walls (cavity_z_0 / cavity_z_l), obstacles with wall_bounce_back, body force,
+ * BGK or MRT. */
+typedef struct hlbm_batch hlbm_batch;
+
+typedef struct {
+  int32_t storage;      /* enum hlbm_storage, one per batch                         */
+  int32_t chunk_steps;  /* steps per hlbm_batch_run call (graph length)             */
+  int32_t use_graph;    /* 1: replay a captured HIP graph per chunk                 */
+  int32_t pad;
+  uint64_t budget_bytes;/* HBM for populations + flags of all resident solves       */
+} hlbm_batch_desc;
+
+typedef struct {
+  int32_t nodes[3];
+  int32_t periodic[3];
+  int32_t collision;    /* enum hlbm_collision                                       */
+  int32_t steps;        /* time steps to run                                         */
+  int32_t wall_bb;      /* wall_bounce_back on the obstacles                         */
+  int32_t n_obstacles;
+  int32_t cavity_on[2]; /* cavity_z_0 / cavity_z_l moving walls (z must be closed)   */
+  int32_t keep_populations; /* also return the final populations (FP64, node layout) */
+  int32_t mrt_custom;
+  double tau;
+  double fext[3];
+  double dx;
+  double init_rho;      /* initial state f_i = init_rho * w_i                        */
+  double cavity_u[2][3];
+  double mrt_rates[10];
+  const hlbm_obstacle* obstacles; /* copied at submit                                */
+} hlbm_solve_desc;
+
+enum hlbm_solve_state { HLBM_SOLVE_QUEUED = 0, HLBM_SOLVE_RUNNING = 1, HLBM_SOLVE_DONE = 2 };
+
+typedef struct {
+  int64_t queued, running, done, chunks;
+  uint64_t bytes_in_use, budget_bytes;
+  double node_updates; /* lattice node updates performed so far                     */
+  double gpu_ms;       /* device time of all chunks (HIP events)                    */
+} hlbm_batch_info;
+
+int hlbm_batch_create(const hlbm_batch_desc* desc, hlbm_batch** out);
+int hlbm_batch_destroy(hlbm_batch* b);
+/* Queue a solve; fails with HLBM_ERR_NOMEM if it can never fit the budget. */
+int hlbm_batch_submit(hlbm_batch* b, const hlbm_solve_desc* s, int64_t* id);
+/* Admit what fits, run one chunk, retire finished solves. */
+int hlbm_batch_run(hlbm_batch* b, int32_t* n_finished);
+int hlbm_batch_status(hlbm_batch* b, int64_t id, int32_t* state, int64_t* steps_done);
+/* Seconds on the library's steady clock: submission, admission, completion. */
+int hlbm_batch_times(hlbm_batch* b, int64_t id, double* t_submit, double* t_admit, double* t_done);
+/* Copy a finished solve's rho (N), u (3N) and, if kept, f (19N), then forget it. */
+int hlbm_batch_result(hlbm_batch* b, int64_t id, double* rho, double* u, double* f);
+int hlbm_batch_info_get(hlbm_batch* b, hlbm_batch_info* out);
+/* The library's steady clock, seconds (same clock as hlbm_batch_times). */
+double hlbm_clock(void);
+
 /* ---- measurement helper (no counterpart in the reference) --------------
  * Times `reps` streaming copies of `bytes` bytes between two scratch buffers
  * it allocates and frees itself (kernel "hbm_copy_kernel"): the achievable

@@ -44,6 +44,9 @@ def test_struct_layout_matches_header(tmp_path):
         " offsetof(hlbm_domain_desc, dx), offsetof(hlbm_domain_desc, mrt_rates), offsetof(hlbm_domain_desc, fext));\n"
         'printf("%zu %zu\\n", sizeof(hlbm_bc), offsetof(hlbm_bc, U));\n'
         'printf("%zu %zu %zu\\n", sizeof(hlbm_obstacle), offsetof(hlbm_obstacle, p1), offsetof(hlbm_obstacle, radius));\n'
+        'printf("%zu %zu %zu %zu\\n", sizeof(hlbm_solve_desc), offsetof(hlbm_solve_desc, tau),'
+        ' offsetof(hlbm_solve_desc, cavity_u), offsetof(hlbm_solve_desc, obstacles));\n'
+        'printf("%zu %zu %zu\\n", sizeof(hlbm_batch_desc), sizeof(hlbm_batch_info), offsetof(hlbm_batch_info, gpu_ms));\n'
         "return 0;}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
@@ -55,6 +58,12 @@ def test_struct_layout_matches_header(tmp_path):
     assert b == [ctypes.sizeof(_lib.BC), _lib.BC.U.offset]
     o = [int(v) for v in lines[2].split()]
     assert o == [ctypes.sizeof(_lib.Obstacle), _lib.Obstacle.p1.offset, _lib.Obstacle.radius.offset]
+    from hippolbm_amd import batch as bt
+    sd = [int(v) for v in lines[3].split()]
+    S = bt.SolveDesc
+    assert sd == [ctypes.sizeof(S), S.tau.offset, S.cavity_u.offset, S.obstacles.offset]
+    bd = [int(v) for v in lines[4].split()]
+    assert bd == [ctypes.sizeof(bt.BatchDesc), ctypes.sizeof(bt.BatchInfo), bt.BatchInfo.gpu_ms.offset]
 
 
 def test_null_arguments_are_rejected_without_a_device():

@@ -1,0 +1,124 @@
+"""Batched domains / solve server: each solve's result is the same alone (the
+single-domain path) and in any mix, with any chunk length, with or without the
+captured graph, and under a memory budget that forces queued admission."""
+import sys
+
+import numpy as np
+import pytest
+
+from hippolbm_amd.batch import Batch, Solve
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def mix(seed, n=6, keep=True):
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        nodes = tuple(int(v) for v in rng.integers(6, 23, size=3))
+        kind = k % 3
+        if kind == 0:  # closed cavity with a moving lid
+            out.append(Solve(nodes=nodes, steps=int(rng.integers(5, 40)), tau=float(rng.uniform(0.52, 1.0)),
+                             cavity_zl=(float(rng.uniform(0.01, 0.1)), 0.0, 0.0), keep_populations=keep))
+        elif kind == 1:  # periodic channel driven by a body force, closed in z
+            out.append(Solve(nodes=nodes, steps=int(rng.integers(5, 40)), tau=float(rng.uniform(0.6, 1.0)),
+                             periodic=(True, True, False), fext=(1e-5, 0.0, 0.0), collision="mrt",
+                             keep_populations=keep))
+        else:  # porous box: balls away from the faces, wall bounce-back
+            balls = [{"kind": "ball", "center": tuple(float(rng.uniform(2, m - 3)) for m in nodes),
+                      "radius": float(rng.uniform(1.0, 2.0))} for _ in range(2)]
+            out.append(Solve(nodes=nodes, steps=int(rng.integers(5, 40)), tau=0.7, fext=(0.0, 2e-5, 0.0),
+                             obstacles=balls, cavity_z0=(0.0, 0.02, 0.0), keep_populations=keep))
+    return out
+
+
+def run_all(b, ids, max_chunks=10000):
+    done = {}
+    chunks = 0
+    while len(done) < len(ids) and chunks < max_chunks:
+        b.run()
+        chunks += 1
+        for sid in ids:
+            if sid not in done and b.state(sid)[0] == 2:
+                done[sid] = b.result(sid)
+    assert len(done) == len(ids), "batch did not finish"
+    return done
+
+
+@pytest.mark.parametrize("storage", ["f64", "f16"])
+def test_batch_equals_single_domain(storage):
+    solves = mix(11)
+    b = Batch(storage=storage, budget_bytes=512 << 20, chunk_steps=8, use_graph=True)
+    ids = [b.submit(s) for s in solves]
+    res = run_all(b, ids)
+    for sid, s in zip(ids, solves):
+        rho, u, f = res[sid]
+        r1, u1, f1 = s.run_single(storage)
+        say(storage, s.nodes, s.steps, "max|df| vs single-domain path", float(np.abs(f - f1).max()))
+        if storage == "f64":  # both paths are built without contraction: bit-identical
+            assert np.array_equal(f, f1)
+            assert np.array_equal(rho, r1)
+            assert np.array_equal(u, u1)
+        else:  # FP32 arithmetic may contract differently in the two kernels
+            assert np.abs(f - f1).max() < 1e-5
+    b.close()
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16", "bf16"])
+def test_alone_equals_in_the_mix(storage):
+    solves = mix(13)
+    b = Batch(storage=storage, budget_bytes=512 << 20, chunk_steps=8)
+    ids = [b.submit(s) for s in solves]
+    res = run_all(b, ids)
+    b.close()
+    for k in (0, 2, 4):
+        a = Batch(storage=storage, budget_bytes=512 << 20, chunk_steps=8)
+        sid = a.submit(solves[k])
+        alone = run_all(a, [sid])[sid]
+        a.close()
+        assert np.array_equal(alone[2], res[ids[k]][2])
+        assert np.array_equal(alone[1], res[ids[k]][1])
+
+
+def test_chunk_length_and_graph_do_not_change_results():
+    solves = mix(12)
+    outs = []
+    for chunk, graph in ((8, True), (5, False), (64, True)):
+        b = Batch(storage="f32", budget_bytes=512 << 20, chunk_steps=chunk, use_graph=graph)
+        ids = [b.submit(s) for s in solves]
+        res = run_all(b, ids)
+        outs.append([res[i][2] for i in ids])
+        b.close()
+    for other in outs[1:]:
+        for a, c in zip(outs[0], other):
+            assert np.array_equal(a, c)
+
+
+def test_admission_under_a_small_budget():
+    # room for about two solves at a time: the rest wait in the queue
+    solves = [Solve(nodes=(20, 20, 20), steps=17 + 3 * k, tau=0.8, cavity_zl=(0.05, 0, 0), keep_populations=True)
+              for k in range(6)]
+    per_solve = 20 ** 3 * (38 * 2 + 1)
+    b = Batch(storage="f16", budget_bytes=int(2.5 * per_solve), chunk_steps=10)
+    ids = [b.submit(s) for s in solves]
+    b.run()
+    info = b.info()
+    assert info["running"] <= 2 and info["queued"] >= 4
+    res = run_all(b, ids)
+    for sid, s in zip(ids, solves):
+        _, _, f1 = s.run_single("f16")
+        assert np.array_equal(res[sid][2], f1)
+    b.close()
+
+
+def test_oversized_solve_is_rejected():
+    b = Batch(storage="f16", budget_bytes=1 << 20, chunk_steps=4)
+    from hippolbm_amd._lib import HLBMError
+    with pytest.raises(HLBMError):
+        b.submit(Solve(nodes=(64, 64, 64), steps=1, tau=0.8))
+    b.close()
