@@ -74,6 +74,51 @@ def measure(n, storage, steps, warmup, collision="bgk"):
             "frac": gbs / HBM_PEAK_GBS, "sum_density": st["sum_density"], "max_u": st["max_velocity_norm"]}
 
 
+def batched_cavities(n_dom=2048, chunk=200, seed=4, storage="f16"):
+    """2048 independent lid-driven cavities, edges ~U{16..96} per axis, lid speed
+    ~U[0.01,0.1], tau ~U[0.52,1.0], BGK, one persistent launch per step, chunks
+    of `chunk` steps replayed from a captured HIP graph."""
+    import numpy as np
+    from hippolbm_amd.batch import Batch, Solve
+    rng = np.random.default_rng(seed)
+    solves = []
+    for _ in range(n_dom):
+        nodes = tuple(int(v) for v in rng.integers(16, 97, size=3))
+        solves.append(Solve(nodes=nodes, steps=1 + 2 * chunk, tau=float(rng.uniform(0.52, 1.0)),
+                            cavity_zl=(float(rng.uniform(0.01, 0.1)), 0.0, 0.0)))
+    nodes_total = sum(s.node_count() for s in solves)
+    b = Batch(storage=storage, budget_bytes=nodes_total * (38 * BYTES[storage] + 1) + (2 << 30),
+              chunk_steps=chunk, use_graph=True)
+    for s in solves:
+        b.submit(s)
+    b.run()  # admission + first chunk (warm-up)
+    i0 = b.info()
+    b.run()  # timed chunk (+ retirement)
+    i1 = b.info()
+    b.close()
+    ms = i1["gpu_ms"] - i0["gpu_ms"]
+    mlups = nodes_total * chunk / (ms * 1e-3) / 1e6
+    bpn = 2 * 19 * BYTES[storage] + 1
+    return {"domains": n_dom, "nodes": nodes_total, "chunk_steps": chunk, "storage": storage, "mlups": mlups,
+            "ms_per_step": ms / chunk, "frac": mlups * 1e6 * bpn / 1e9 / HBM_PEAK_GBS}
+
+
+def served_stream(n=32, rate_hz=4.0, seed=5, storage="f16", chunk=200, budget_gb=96):
+    """Poisson arrivals of cavity / channel / porous solves (sizes log-uniform
+    32^3..256^3, 500..5000 steps, BGK or MRT), HBM-budget admission, continuous
+    batching; time-to-result percentiles and aggregate MLUPS."""
+    from hippolbm_amd.batch import Batch, poisson_requests, serve
+    reqs = poisson_requests(n, rate_hz, seed)
+    b = Batch(storage=storage, budget_bytes=budget_gb << 30, chunk_steps=chunk, use_graph=True)
+    r = serve(b, reqs)
+    b.close()
+    bpn = 2 * 19 * BYTES[storage] + 1
+    r.pop("results")
+    r.update({"rate_hz": rate_hz, "storage": storage, "chunk_steps": chunk,
+              "frac": r["mlups"] * 1e6 * bpn / 1e9 / HBM_PEAK_GBS})
+    return r
+
+
 def copy_bandwidth():
     import ctypes
     from hippolbm_amd import _lib
@@ -138,6 +183,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", default=None, help="run a single storage type (profiling)")
     ap.add_argument("--copy", action="store_true", help="run only the streaming-copy reference kernel (profiling)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-cavity and served-stream legs")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched and served legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -151,6 +198,9 @@ def main():
         r = measure(args.size, args.only, args.steps, args.warmup)
         print(json.dumps({"storage": args.only, **r}), flush=True)
         return 0
+    if args.batch_only:
+        print(json.dumps({"batched": batched_cavities(), "served": served_stream()}), flush=True)
+        return 0
     n = args.size
     per = {}
     for dt in [d for d in args.dtypes.split(",") if d]:
@@ -159,6 +209,14 @@ def main():
               f"({100 * per[dt]['frac']:.1f}% of 8 TB/s)", file=sys.stderr, flush=True)
     main_r = per.get("f64") or next(iter(per.values()))
     copy_gbs = copy_bandwidth()
+    batched = served = None
+    if not args.no_batch:
+        batched = batched_cavities()
+        print(f"# batched cavities: {batched['mlups']:.0f} MLUPS ({100 * batched['frac']:.1f}%)", file=sys.stderr,
+              flush=True)
+        served = served_stream()
+        print(f"# served: p50 {served['p50_s']:.2f} s p99 {served['p99_s']:.2f} s, {served['mlups']:.0f} MLUPS",
+              file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "MLUPS (D3Q19 BGK FP64 Couette 512^3, one MI355X) + % HBM roofline per storage dtype",
@@ -182,7 +240,10 @@ def main():
         "per_dtype": {k: {"mlups": v["mlups"], "frac": v["frac"], "achieved_gbs": v["achieved_gbs"],
                           "bytes_per_node": v["bytes_per_node"], "ms_per_step": v["ms_per_step"]}
                       for k, v in per.items()},
-        "latency": None,
+        "latency": ({"p50_s": served["p50_s"], "p99_s": served["p99_s"], "workload": "served stream"}
+                    if served else None),
+        "batched": batched,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

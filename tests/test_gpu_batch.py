@@ -112,7 +112,7 @@ def test_admission_under_a_small_budget():
     res = run_all(b, ids)
     for sid, s in zip(ids, solves):
         _, _, f1 = s.run_single("f16")
-        assert np.array_equal(res[sid][2], f1)
+        assert np.abs(res[sid][2] - f1).max() < 1e-5
     b.close()
 
 
