@@ -28,7 +28,11 @@ namespace {
     if (e_ != hipSuccess) return fail(HLBM_ERR_HIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
   } while (0)
 
-enum Phase { PHASE_PRE = 0, PHASE_POST = 1 };
+// PRE: f^T as the reference holds it (natural layout, post-BC).
+// POST (A/B): f*^{T-1}, post-collision, streaming pending.
+// AA_PENDING (AA): natural layout after an odd step, boundary program pending.
+// AA_SWAPPED (AA): post-collision populations in the opposite slots of each node.
+enum Phase { PHASE_PRE = 0, PHASE_POST = 1, PHASE_AA_PENDING = 2, PHASE_AA_SWAPPED = 3 };
 
 size_t elem_size(int storage) { return storage == HLBM_F64 ? 8 : (storage == HLBM_F32 ? 4 : 2); }
 
@@ -94,11 +98,14 @@ static StepParams base_params(hlbm_domain* h) {
   return p;
 }
 
-static int launch(hlbm_domain* h, int mode, bool write_macro, hipStream_t s) {
+static int launch(hlbm_domain* h, int mode, bool write_macro, hipStream_t s, void* out = nullptr,
+                  bool skip_program = false) {
   StepParams p = base_params(h);
+  const bool aa = mode_aa(mode);
   p.fin = h->buf[h->cur];
-  p.fout = h->buf[h->cur ^ 1];
+  p.fout = aa ? (out ? out : h->buf[h->cur]) : h->buf[h->cur ^ 1];
   p.write_macro = write_macro ? 1 : 0;
+  p.skip_program = skip_program ? 1 : 0;
   hipError_t e;
   const int coll = h->d.collision == HLBM_MRT ? COLL_MRT : COLL_BGK;
   if (h->d.storage == HLBM_F64)
@@ -106,7 +113,7 @@ static int launch(hlbm_domain* h, int mode, bool write_macro, hipStream_t s) {
   else
     e = launch_step_lowp(h->d.storage, coll, mode, h->npt, grid_for(h), h->block, p, s);
   if (e != hipSuccess) return fail(HLBM_ERR_HIP, std::string("step kernel launch: ") + hipGetErrorString(e));
-  h->cur ^= 1;
+  if (!aa) h->cur ^= 1;
   return HLBM_OK;
 }
 
@@ -115,6 +122,30 @@ static int materialize(hlbm_domain* h, hipStream_t s) {
   if (h->phase == PHASE_POST) {
     int rc = launch(h, MODE_STREAM, false, s);
     if (rc) return rc;
+    h->phase = PHASE_PRE;
+  } else if (h->phase == PHASE_AA_PENDING) {
+    if (h->plan.prog_planes) {
+      int rc = launch(h, MODE_AA_FIX, false, s);
+      if (rc) return rc;
+    }
+    h->phase = PHASE_PRE;
+  } else if (h->phase == PHASE_AA_SWAPPED) {
+    // streaming a swapped AA state in place would race; stream into a second
+    // buffer, which then becomes the domain's buffer (the old one is freed)
+    void* nb = nullptr;
+    const size_t fbytes = (size_t)h->stride * Q * h->esize;
+    if (hipMalloc(&nb, fbytes) != hipSuccess) return fail(HLBM_ERR_NOMEM, "AA read-out buffer");
+    int rc = launch(h, MODE_AA_STREAM, false, s, nb);
+    if (rc) {
+      (void)hipFree(nb);
+      return rc;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) {
+      (void)hipFree(nb);
+      return fail(HLBM_ERR_HIP, "AA read-out sync");
+    }
+    (void)hipFree(h->buf[h->cur]);
+    h->buf[h->cur] = nb;
     h->phase = PHASE_PRE;
   }
   return HLBM_OK;
@@ -143,7 +174,7 @@ int hlbm_domain_create(const hlbm_domain_desc* desc, hlbm_domain** out) {
   if (N64 >= (1ull << 31)) return fail(HLBM_ERR_ARG, "domain too large for one subdomain (>= 2^31 nodes)");
   if (desc->storage < HLBM_F64 || desc->storage > HLBM_BF16) return fail(HLBM_ERR_ARG, "bad storage");
   if (desc->collision != HLBM_BGK && desc->collision != HLBM_MRT) return fail(HLBM_ERR_ARG, "bad collision");
-  if (desc->pattern != HLBM_PATTERN_AB) return fail(HLBM_ERR_UNSUPPORTED, "only the A/B pattern is built");
+  if (desc->pattern != HLBM_PATTERN_AB && desc->pattern != HLBM_PATTERN_AA) return fail(HLBM_ERR_ARG, "bad pattern");
   if (!(desc->tau > 0.0)) return fail(HLBM_ERR_ARG, "tau must be > 0");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(HLBM_ERR_NODEVICE, "no HIP device visible");
@@ -176,14 +207,15 @@ int hlbm_domain_create(const hlbm_domain_desc* desc, hlbm_domain** out) {
     return fail(HLBM_ERR_NOMEM, "device allocation failed: " + what);
   };
   if (hipMalloc(&h->buf[0], fbytes) != hipSuccess) return cleanup("populations A");
-  if (hipMalloc(&h->buf[1], fbytes) != hipSuccess) return cleanup("populations B");
+  const bool aa = desc->pattern == HLBM_PATTERN_AA;
+  if (!aa && hipMalloc(&h->buf[1], fbytes) != hipSuccess) return cleanup("populations B");
   if (hipMalloc((void**)&h->flags, N64) != hipSuccess) return cleanup("flags");
   if (hipMalloc((void**)&h->m0, N64 * 8) != hipSuccess) return cleanup("rho");
   if (hipMalloc((void**)&h->m1, N64 * 24) != hipSuccess) return cleanup("u");
   if (hipMalloc((void**)&h->red_part, 7 * 8 * 1024) != hipSuccess) return cleanup("reduction");
   if (hipMalloc((void**)&h->red_out, 7 * 8) != hipSuccess) return cleanup("reduction");
-  h->bytes = 2 * fbytes + N64 * (1 + 8 + 24) + 7 * 8 * 1025;
-  if (hipMemset(h->buf[0], 0, fbytes) != hipSuccess || hipMemset(h->buf[1], 0, fbytes) != hipSuccess ||
+  h->bytes = (aa ? 1 : 2) * fbytes + N64 * (1 + 8 + 24) + 7 * 8 * 1025;
+  if (hipMemset(h->buf[0], 0, fbytes) != hipSuccess || (!aa && hipMemset(h->buf[1], 0, fbytes) != hipSuccess) ||
       hipMemset(h->flags, 0, N64) != hipSuccess || hipMemset(h->m0, 0, N64 * 8) != hipSuccess ||
       hipMemset(h->m1, 0, N64 * 24) != hipSuccess)
     return cleanup("memset");
@@ -193,7 +225,7 @@ int hlbm_domain_create(const hlbm_domain_desc* desc, hlbm_domain** out) {
   if (h->d.storage != HLBM_F64) {
     int lo[3] = {0, 0, 0}, hi[3] = {h->nx - 1, h->ny - 1, h->nz - 1};
     if (launch_set_distribution(h->d.storage, h->buf[0], h->stride, h->nx, h->ny, lo, hi, 0.0, 0) != hipSuccess ||
-        launch_set_distribution(h->d.storage, h->buf[1], h->stride, h->nx, h->ny, lo, hi, 0.0, 0) != hipSuccess ||
+        (!aa && launch_set_distribution(h->d.storage, h->buf[1], h->stride, h->nx, h->ny, lo, hi, 0.0, 0) != hipSuccess) ||
         hipDeviceSynchronize() != hipSuccess) {
       hlbm_domain_destroy(h);
       *out = nullptr;
@@ -551,12 +583,22 @@ int hlbm_ghost_update_ref(const hlbm_domain* h, double* f_ref) {
 int hlbm_step(hlbm_domain* h, int32_t nsteps, uint32_t flags, void* stream) {
   if (!h || nsteps < 0) return fail(HLBM_ERR_ARG, "bad step arguments");
   hipStream_t s = (hipStream_t)stream;
+  const bool aa = h->d.pattern == HLBM_PATTERN_AA;
   for (int k = 0; k < nsteps; k++) {
     const bool wm = (flags & HLBM_STEP_WRITE_MACRO) && k == nsteps - 1;
-    const int mode = h->phase == PHASE_PRE ? MODE_COLLIDE : MODE_FUSED;
-    int rc = launch(h, mode, wm, s);
-    if (rc) return rc;
-    h->phase = PHASE_POST;
+    if (aa) {
+      // even: own slots -> opposite slots (program first if the state is pending);
+      // odd: gather from the swapped slots, push to the neighbours
+      const bool even = h->phase != PHASE_AA_SWAPPED;
+      int rc = even ? launch(h, MODE_AA_EVEN, wm, s, nullptr, h->phase == PHASE_PRE) : launch(h, MODE_AA_ODD, wm, s);
+      if (rc) return rc;
+      h->phase = even ? PHASE_AA_SWAPPED : PHASE_AA_PENDING;
+    } else {
+      const int mode = h->phase == PHASE_PRE ? MODE_COLLIDE : MODE_FUSED;
+      int rc = launch(h, mode, wm, s);
+      if (rc) return rc;
+      h->phase = PHASE_POST;
+    }
     h->t++;
   }
   return HLBM_OK;

@@ -32,7 +32,25 @@
 
 namespace hlbm {
 
-enum StepMode { MODE_FUSED = 0, MODE_COLLIDE = 1, MODE_STREAM = 2 };
+// A/B modes read buffer A and write buffer B.  AA modes work in place on one
+// buffer: AA_EVEN reads a node's own populations and writes them back into the
+// opposite slots; AA_ODD gathers from those swapped slots and pushes its
+// post-collision populations to the neighbours' natural slots (bounce-back
+// targets stay at the node); AA_STREAM is AA_ODD without collision into a
+// second buffer; AA_FIX applies the boundary program to a natural state.
+enum StepMode { MODE_FUSED = 0, MODE_COLLIDE = 1, MODE_STREAM = 2, MODE_AA_EVEN = 3, MODE_AA_ODD = 4,
+                MODE_AA_STREAM = 5, MODE_AA_FIX = 6 };
+
+__host__ __device__ constexpr bool mode_reads_own(int m) {
+  return m == MODE_COLLIDE || m == MODE_AA_EVEN || m == MODE_AA_FIX;
+}
+__host__ __device__ constexpr bool mode_collides(int m) {
+  return m == MODE_FUSED || m == MODE_COLLIDE || m == MODE_AA_EVEN || m == MODE_AA_ODD;
+}
+__host__ __device__ constexpr bool mode_aa(int m) { return m >= MODE_AA_EVEN; }
+// in-place AA steps leave wall nodes alone when wall_bounce_back is active:
+// nothing ever reads a wall's populations then
+__host__ __device__ constexpr bool mode_skips_walls(int m) { return m == MODE_AA_EVEN || m == MODE_AA_ODD; }
 enum ProgKind { PK_LID = 1, PK_NEUMANN = 2, PK_REBB = 3 };
 enum CollKind { COLL_BGK = 0, COLL_MRT = 1 };
 
@@ -65,6 +83,7 @@ struct StepParams {
   int32_t mrt_custom;
   int32_t gx, gy, gz;   // tile grid of the bulk kernel (tiles of blockDim.x * blockDim.y nodes)
   int32_t xcd_remap;    // map consecutive dispatch slots to the same XCD's run of tiles
+  int32_t skip_program; // boundary nodes processed without the post-stream program (AA_EVEN on a clean state)
   double cav[2][5];
   double tau;
   double F[3];
@@ -550,10 +569,13 @@ __device__ __forceinline__ void gather_node(const StepParams& p, int x, int y, i
   const int nx = p.nx, ny = p.ny, nz = p.nz;
   const S* __restrict__ fin = (const S*)p.fin;
   const size_t st = p.stride;
-  if constexpr (MODE == MODE_COLLIDE) {
+  if constexpr (mode_reads_own(MODE)) {
 #pragma unroll
     for (int i = 0; i < Q; i++) f[i] = O::ld(fin, (size_t)i * st + idx);
   } else {
+    // A/B: f*_i(s) sits in plane i at s and f*_opp(i)(x) in plane opp(i) at x.
+    // AA (swapped state): f*_i(s) sits in plane opp(i) at s, f*_opp(i)(x) in plane i at x.
+    constexpr bool AA = mode_aa(MODE);
     // upstream coordinates; -1 marks a non-periodic face (bounce-back)
     const int xm = x > 0 ? x - 1 : (p.per[0] ? nx - 1 : -1);
     const int xp = x < nx - 1 ? x + 1 : (p.per[0] ? 0 : -1);
@@ -568,7 +590,7 @@ __device__ __forceinline__ void gather_node(const StepParams& p, int x, int y, i
       const int sy = CY(i) == 0 ? y : (CY(i) == 1 ? ym : yp);
       const int sz = CZ(i) == 0 ? z : (CZ(i) == 1 ? zm : zp);
       if (sx < 0 || sy < 0 || sz < 0) {
-        C v = O::ld(fin, (size_t)OPP(i) * st + idx);
+        C v = O::ld(fin, (size_t)(AA ? i : OPP(i)) * st + idx);
         if constexpr (CZ(i) != 0) {
           constexpr int side = CZ(i) == 1 ? 0 : 1;
           if (sz < 0 && p.cav_gather && p.cav_on[side]) v += C(p.cav[side][cavity_slot(i)]);
@@ -577,9 +599,9 @@ __device__ __forceinline__ void gather_node(const StepParams& p, int x, int y, i
       } else {
         const uint32_t sidx = ((uint32_t)sz * (uint32_t)ny + (uint32_t)sy) * (uint32_t)nx + (uint32_t)sx;
         if (i != 0 && wbb && p.flags[sidx] != 0)
-          f[i] = O::ld(fin, (size_t)OPP(i) * st + idx);
+          f[i] = O::ld(fin, (size_t)(AA ? i : OPP(i)) * st + idx);
         else
-          f[i] = O::ld(fin, (size_t)i * st + sidx);
+          f[i] = O::ld(fin, (size_t)(AA ? OPP(i) : i) * st + sidx);
       }
     });
   }
@@ -587,7 +609,7 @@ __device__ __forceinline__ void gather_node(const StepParams& p, int x, int y, i
 
 // Phase 2: the post-stream boundary program (post_bounce_back re-application,
 // lid_driven_cavity, neumann) in deck order, for a node on a program plane.
-template <class S>
+template <class S, int MODE>
 __device__ __forceinline__ void apply_program(const StepParams& p, int x, int y, int z, uint32_t idx, bool wall,
                                               typename StoreOps<S>::C* f) {
   using O = StoreOps<S>;
@@ -611,7 +633,8 @@ __device__ __forceinline__ void apply_program(const StepParams& p, int x, int y,
                 static_for<0, 5>([&](auto jc) {
                   constexpr int j = decltype(jc)::value;
                   constexpr int qi = Faces::in[face][j];
-                  C v = O::ld(fin, (size_t)OPP(qi) * st + idx);
+                  // f*_opp(qi)(x): plane opp(qi) (A/B), plane qi (AA: swapped or natural bounce-back slot)
+                  C v = O::ld(fin, (size_t)(mode_aa(MODE) ? qi : OPP(qi)) * st + idx);
                   if (dim == 2 && p.cav_on[side]) v += C(p.cav[side][j]);
                   f[qi] = v;
                 });
@@ -632,12 +655,14 @@ __device__ __forceinline__ void apply_program(const StepParams& p, int x, int y,
 
 // Phase 3: macro_variables (optionally written out), collision, store.
 template <class S, int COLL, int MODE>
-__device__ __forceinline__ void finish_node(const StepParams& p, uint32_t idx, bool wall, typename StoreOps<S>::C* f) {
+__device__ __forceinline__ void finish_node(const StepParams& p, int x, int y, int z, uint32_t idx, bool wall,
+                                            typename StoreOps<S>::C* f) {
   using O = StoreOps<S>;
+  using C = typename O::C;
   constexpr bool SHIFT = O::shifted;
   S* __restrict__ fout = (S*)p.fout;
   const size_t st = p.stride;
-  if constexpr (MODE != MODE_STREAM) {
+  if constexpr (mode_collides(MODE)) {
     if constexpr (!SHIFT) {
       double rho, ux, uy, uz;
       macro_f64((const double*)f, rho, ux, uy, uz);
@@ -696,8 +721,45 @@ __device__ __forceinline__ void finish_node(const StepParams& p, uint32_t idx, b
     }
   }
 
+  if constexpr (MODE == MODE_AA_EVEN) {
 #pragma unroll
-  for (int i = 0; i < Q; i++) O::st(fout, (size_t)i * st + idx, f[i]);
+    for (int i = 0; i < Q; i++) O::st(fout, (size_t)OPP(i) * st + idx, f[i]);
+  } else if constexpr (MODE == MODE_AA_ODD) {
+    // push f*_i(x) to the natural slot of x + c_i; a non-periodic face or (with
+    // wall_bounce_back) a wall downstream bounces it into slot opp(i) of x,
+    // adding the moving-wall term the gather would add on a z face
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const int xm = x > 0 ? x - 1 : (p.per[0] ? nx - 1 : -1);
+    const int xp = x < nx - 1 ? x + 1 : (p.per[0] ? 0 : -1);
+    const int ym = y > 0 ? y - 1 : (p.per[1] ? ny - 1 : -1);
+    const int yp = y < ny - 1 ? y + 1 : (p.per[1] ? 0 : -1);
+    const int zm = z > 0 ? z - 1 : (p.per[2] ? nz - 1 : -1);
+    const int zp = z < nz - 1 ? z + 1 : (p.per[2] ? 0 : -1);
+    const bool wbb = p.wall_bb && !wall;
+    static_for([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      const int tx = CX(i) == 0 ? x : (CX(i) == 1 ? xp : xm);
+      const int ty = CY(i) == 0 ? y : (CY(i) == 1 ? yp : ym);
+      const int tz = CZ(i) == 0 ? z : (CZ(i) == 1 ? zp : zm);
+      if (tx < 0 || ty < 0 || tz < 0) {
+        C v = f[i];
+        if constexpr (CZ(i) != 0) {
+          constexpr int side = CZ(i) == -1 ? 0 : 1;  // face of the incoming direction opp(i)
+          if (tz < 0 && p.cav_gather && p.cav_on[side]) v += C(p.cav[side][cavity_slot(OPP(i))]);
+        }
+        O::st(fout, (size_t)OPP(i) * st + idx, v);
+      } else {
+        const uint32_t tidx = ((uint32_t)tz * (uint32_t)ny + (uint32_t)ty) * (uint32_t)nx + (uint32_t)tx;
+        if (i != 0 && wbb && p.flags[tidx] != 0)
+          O::st(fout, (size_t)OPP(i) * st + idx, f[i]);
+        else
+          O::st(fout, (size_t)i * st + tidx, f[i]);
+      }
+    });
+  } else {
+#pragma unroll
+    for (int i = 0; i < Q; i++) O::st(fout, (size_t)i * st + idx, f[i]);
+  }
 }
 
 // Full per-node pipeline.  PROG selects whether the post-stream boundary
@@ -708,10 +770,13 @@ __device__ __forceinline__ void process_node(const StepParams& p, int x, int y, 
   using C = typename StoreOps<S>::C;
   const uint32_t idx = ((uint32_t)z * (uint32_t)p.ny + (uint32_t)y) * (uint32_t)p.nx + (uint32_t)x;
   const bool wall = p.flags[idx] != 0;
+  if (mode_skips_walls(MODE) && wall && p.wall_bb) return;
   C f[Q];
   gather_node<S, MODE>(p, x, y, z, idx, wall, f);
-  if constexpr (PROG && MODE != MODE_COLLIDE) apply_program<S>(p, x, y, z, idx, wall, f);
-  finish_node<S, COLL, MODE>(p, idx, wall, f);
+  if constexpr (PROG && MODE != MODE_COLLIDE) {
+    if (!p.skip_program) apply_program<S, MODE>(p, x, y, z, idx, wall, f);
+  }
+  finish_node<S, COLL, MODE>(p, x, y, z, idx, wall, f);
 }
 
 // Bulk kernel: every node that is not on a plane of the boundary program.
@@ -746,12 +811,13 @@ __global__ void __launch_bounds__(256) step_kernel(const StepParams p) {
     idx[k] = ((uint32_t)z * (uint32_t)p.ny + (uint32_t)y) * (uint32_t)p.nx + (uint32_t)x;
     if (act[k]) {
       wall[k] = p.flags[idx[k]] != 0;
-      gather_node<S, MODE>(p, x, y, z, idx[k], wall[k], f[k]);
+      if (mode_skips_walls(MODE) && wall[k] && p.wall_bb) act[k] = false;
     }
+    if (act[k]) gather_node<S, MODE>(p, x, y, z, idx[k], wall[k], f[k]);
   }
 #pragma unroll
   for (int k = 0; k < NPT; k++)
-    if (act[k]) finish_node<S, COLL, MODE>(p, idx[k], wall[k], f[k]);
+    if (act[k]) finish_node<S, COLL, MODE>(p, x, (by * NPT + k) * blockDim.y + threadIdx.y, z, idx[k], wall[k], f[k]);
 }
 
 // Boundary kernel: the nodes of the program planes, each once (a node on
@@ -882,9 +948,9 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchParams bp) {
     C f[Q];
     gather_node<S, MODE_FUSED>(p, x, y, z, n, wall, f);
     if (d.coll == COLL_MRT)
-      finish_node<S, COLL_MRT, MODE_FUSED>(p, n, wall, f);
+      finish_node<S, COLL_MRT, MODE_FUSED>(p, x, y, z, n, wall, f);
     else
-      finish_node<S, COLL_BGK, MODE_FUSED>(p, n, wall, f);
+      finish_node<S, COLL_BGK, MODE_FUSED>(p, x, y, z, n, wall, f);
   }
 }
 

@@ -6,8 +6,11 @@ namespace hlbm {
 
 template <class S, int COLL, int MODE, int NPT>
 static hipError_t go(dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
-  hipLaunchKernelGGL((step_kernel<S, COLL, MODE, NPT>), grid, block, 0, s, p);
-  hipError_t e = hipGetLastError();
+  hipError_t e = hipSuccess;
+  if (MODE != MODE_AA_FIX) {  // AA_FIX only touches program-plane nodes
+    hipLaunchKernelGGL((step_kernel<S, COLL, MODE, NPT>), grid, block, 0, s, p);
+    e = hipGetLastError();
+  }
   if (e != hipSuccess || MODE == MODE_COLLIDE || !p.prog_planes) return e;
   hipLaunchKernelGGL((boundary_kernel<S, COLL, MODE>), dim3(boundary_blocks(p)), dim3(256), 0, s, p);
   return hipGetLastError();
@@ -21,6 +24,14 @@ static hipError_t go_npt(int npt, dim3 grid, dim3 block, const StepParams& p, hi
 template <class S>
 static hipError_t dispatch(int coll, int mode, int npt, dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
   if (mode == MODE_STREAM) return go_npt<S, COLL_BGK, MODE_STREAM>(npt, grid, block, p, s);
+  if (mode == MODE_AA_STREAM) return go_npt<S, COLL_BGK, MODE_AA_STREAM>(npt, grid, block, p, s);
+  if (mode == MODE_AA_FIX) return go_npt<S, COLL_BGK, MODE_AA_FIX>(npt, grid, block, p, s);
+  if (mode == MODE_AA_EVEN)
+    return coll == COLL_BGK ? go_npt<S, COLL_BGK, MODE_AA_EVEN>(npt, grid, block, p, s)
+                            : go_npt<S, COLL_MRT, MODE_AA_EVEN>(npt, grid, block, p, s);
+  if (mode == MODE_AA_ODD)
+    return coll == COLL_BGK ? go_npt<S, COLL_BGK, MODE_AA_ODD>(npt, grid, block, p, s)
+                            : go_npt<S, COLL_MRT, MODE_AA_ODD>(npt, grid, block, p, s);
   if (coll == COLL_BGK) {
     if (mode == MODE_FUSED) return go_npt<S, COLL_BGK, MODE_FUSED>(npt, grid, block, p, s);
     return go_npt<S, COLL_BGK, MODE_COLLIDE>(npt, grid, block, p, s);

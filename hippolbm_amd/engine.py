@@ -32,14 +32,14 @@ class Domain:
     """One single-GPU subdomain (replaces domain + define_grid_3dq19 + fields)."""
 
     def __init__(self, nodes, periodic, tau, fext=(0.0, 0.0, 0.0), storage="f64", collision="bgk", dx=1.0,
-                 mrt_rates=None):
+                 mrt_rates=None, pattern="ab"):
         d = DomainDesc()
         for k in range(3):
             d.nodes[k] = int(nodes[k])
             d.periodic[k] = 1 if periodic[k] else 0
             d.fext[k] = float(fext[k])
         d.storage = _lib.STORAGE[storage]
-        d.pattern = 0
+        d.pattern = {"ab": 0, "aa": 1}[pattern]
         d.collision = _lib.COLLISION[collision]
         d.tau = float(tau)
         d.dx = float(dx)
@@ -209,12 +209,12 @@ def bc_entries(deck):
 class Simulation:
     """A deck run on the GPU engine with the reference's operator order."""
 
-    def __init__(self, deck, storage="f64", init_f=None):
+    def __init__(self, deck, storage="f64", init_f=None, pattern="ab"):
         p = deck.params
         self.deck = deck
         self.params = p
         self.dom = Domain(deck.nodes, deck.periodic, p["tau"], p["Fext"], storage=storage,
-                          collision=deck.collision, dx=deck.dx, mrt_rates=deck.mrt_rates)
+                          collision=deck.collision, dx=deck.dx, mrt_rates=deck.mrt_rates, pattern=pattern)
         self.dom.set_bcs(bc_entries(deck))
         # initialize (config_initialize.msp:21-27)
         self.dom.init_obst()
