@@ -58,8 +58,9 @@ def time_steps(dom, steps, warmup):
     return ms, wall
 
 
-def measure(n, storage, steps, warmup, collision="bgk"):
+def measure(n, storage, steps, warmup, collision="bgk", tune=True):
     dom = couette_domain(n, storage, collision)
+    tuned = dom.autotune(reps=3) if tune else None
     ms, wall = time_steps(dom, steps, warmup)
     # sanity: the run stayed finite and mass-conserving
     dom.macro()
@@ -71,7 +72,29 @@ def measure(n, storage, steps, warmup, collision="bgk"):
     bpn = 2 * 19 * BYTES[storage] + 1
     gbs = nodes * bpn / (ms_step * 1e-3) / 1e9
     return {"mlups": mlups, "ms_per_step": ms_step, "wall_s": wall, "bytes_per_node": bpn, "achieved_gbs": gbs,
-            "frac": gbs / HBM_PEAK_GBS, "sum_density": st["sum_density"], "max_u": st["max_velocity_norm"]}
+            "frac": gbs / HBM_PEAK_GBS, "sum_density": st["sum_density"], "max_u": st["max_velocity_norm"],
+            "launch": tuned}
+
+
+def porous_channel(steps=100, warmup=10, storage="f64"):
+    """The porous channel 256x128x128 (MRT, s_nu = 1/0.7, other rates 1.2;
+    density 1.001 inlet, zero-gradient outflow, bounce-back y/z walls, 150 seeded
+    non-overlapping spheres r ~ U[5,12]) on the A/B kernel."""
+    from hippolbm_amd import workloads
+    from hippolbm_amd.engine import Simulation
+    d = workloads.porous_channel()
+    sim = Simulation(d, storage=storage)
+    tuned = sim.dom.autotune(reps=3)
+    ms, _ = time_steps(sim.dom, steps, warmup)
+    flags = sim.dom.get_flags()
+    nodes = sim.dom.N
+    sim.dom.close()
+    ms_step = ms / steps
+    bpn = 2 * 19 * BYTES[storage] + 1
+    gbs = nodes * bpn / (ms_step * 1e-3) / 1e9
+    return {"workload": "porous_channel_256x128x128_mrt", "storage": storage, "nodes": nodes,
+            "solid_nodes": int((flags == -2).sum()), "mlups": nodes * steps / (ms * 1e-3) / 1e6,
+            "ms_per_step": ms_step, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS, "launch": tuned}
 
 
 def batched_cavities(n_dom=2048, chunk=200, seed=4, storage="f16"):
@@ -103,6 +126,11 @@ def batched_cavities(n_dom=2048, chunk=200, seed=4, storage="f16"):
             "ms_per_step": ms / chunk, "frac": mlups * 1e6 * bpn / 1e9 / HBM_PEAK_GBS}
 
 
+def served_sweep(rates=(4.0, 16.0, 0.0), n=32, seed=5, storage="f16", chunk=200, budget_gb=96):
+    """served_stream at several arrival rates (0 = every request at t = 0)."""
+    return [served_stream(n, r, seed, storage, chunk, budget_gb) for r in rates]
+
+
 def served_stream(n=32, rate_hz=4.0, seed=5, storage="f16", chunk=200, budget_gb=96):
     """Poisson arrivals of cavity / channel / porous solves (sizes log-uniform
     32^3..256^3, 500..5000 steps, BGK or MRT), HBM-budget admission, continuous
@@ -117,6 +145,43 @@ def served_stream(n=32, rate_hz=4.0, seed=5, storage="f16", chunk=200, budget_gb
     r.update({"rate_hz": rate_hz, "storage": storage, "chunk_steps": chunk,
               "frac": r["mlups"] * 1e6 * bpn / 1e9 / HBM_PEAK_GBS})
     return r
+
+
+def precision_sweep(steps=10000, which=("couette", "poiseuille")):
+    """FP64 vs FP32 compute on FP32/FP16/BF16 shifted storage after `steps`
+    steps: rho and u errors (relative L2 and max) per dtype, for Couette 512^3
+    and Poiseuille 512x256x256 (F = 1e-6, tau = 0.6)."""
+    import numpy as np
+    from hippolbm_amd import workloads
+    from hippolbm_amd.engine import Simulation
+    out = {}
+    decks = {"couette": lambda: workloads.couette((512, 512, 512)),
+             "poiseuille": lambda: workloads.poiseuille((512, 256, 256))}
+    for name in which:
+        d = decks[name]()
+        ref = None
+        res = {}
+        for dt in ("f64", "f32", "f16", "bf16"):
+            t0 = time.perf_counter()
+            sim = Simulation(d, storage=dt)
+            sim.run(steps)
+            rho, u = sim.dom.macro()
+            sim.dom.close()
+            if ref is None:
+                ref = (rho, u)
+                res[dt] = {"u_max": float(np.sqrt((u * u).sum(0)).max()), "rho_mean": float(rho.mean())}
+            else:
+                du = u - ref[1]
+                dr = rho - ref[0]
+                res[dt] = {"u_rel_l2": float(np.sqrt((du * du).sum() / (ref[1] * ref[1]).sum())),
+                           "u_max_abs": float(np.abs(du).max()),
+                           "rho_rel_l2": float(np.sqrt((dr * dr).sum() / (ref[0] * ref[0]).sum())),
+                           "rho_max_abs": float(np.abs(dr).max())}
+            print(f"# precision {name} {dt}: {res[dt]} ({time.perf_counter() - t0:.0f} s)", file=sys.stderr,
+                  flush=True)
+            del rho, u
+        out[name] = {"nodes": list(d.nodes), "steps": steps, "errors_vs_f64": res}
+    return out
 
 
 def copy_bandwidth():
@@ -185,6 +250,9 @@ def main():
     ap.add_argument("--copy", action="store_true", help="run only the streaming-copy reference kernel (profiling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-cavity and served-stream legs")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched and served legs")
+    ap.add_argument("--precision", type=int, default=0, metavar="STEPS",
+                    help="run only the precision sweep (Couette 512^3, Poiseuille 512x256x256) for STEPS steps")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -194,29 +262,36 @@ def main():
         print(json.dumps({"copy_kernel_gbs": copy_bandwidth(), "bytes_read_per_launch": 4 << 30,
                           "bytes_written_per_launch": 4 << 30}), flush=True)
         return 0
+    if args.precision:
+        print(json.dumps({"precision_sweep": precision_sweep(args.precision)}), flush=True)
+        return 0
     if args.only:
-        r = measure(args.size, args.only, args.steps, args.warmup)
+        r = measure(args.size, args.only, args.steps, args.warmup, tune=not args.no_tune)
         print(json.dumps({"storage": args.only, **r}), flush=True)
         return 0
     if args.batch_only:
-        print(json.dumps({"batched": batched_cavities(), "served": served_stream()}), flush=True)
+        print(json.dumps({"batched": batched_cavities(), "served": served_sweep()}), flush=True)
         return 0
     n = args.size
     per = {}
     for dt in [d for d in args.dtypes.split(",") if d]:
-        per[dt] = measure(n, dt, args.steps, args.warmup)
+        per[dt] = measure(n, dt, args.steps, args.warmup, tune=not args.no_tune)
         print(f"# {dt}: {per[dt]['mlups']:.0f} MLUPS, {per[dt]['achieved_gbs']:.0f} GB/s "
               f"({100 * per[dt]['frac']:.1f}% of 8 TB/s)", file=sys.stderr, flush=True)
     main_r = per.get("f64") or next(iter(per.values()))
     copy_gbs = copy_bandwidth()
+    porous = porous_channel()
+    print(f"# porous channel MRT f64: {porous['mlups']:.0f} MLUPS ({100 * porous['frac']:.1f}%)", file=sys.stderr,
+          flush=True)
     batched = served = None
     if not args.no_batch:
         batched = batched_cavities()
         print(f"# batched cavities: {batched['mlups']:.0f} MLUPS ({100 * batched['frac']:.1f}%)", file=sys.stderr,
               flush=True)
-        served = served_stream()
-        print(f"# served: p50 {served['p50_s']:.2f} s p99 {served['p99_s']:.2f} s, {served['mlups']:.0f} MLUPS",
-              file=sys.stderr, flush=True)
+        served = served_sweep()
+        for r in served:
+            print(f"# served @ {r['rate_hz']} Hz: p50 {r['p50_s']:.2f} s p99 {r['p99_s']:.2f} s, "
+                  f"{r['mlups']:.0f} MLUPS ({100 * r['frac']:.1f}%)", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "MLUPS (D3Q19 BGK FP64 Couette 512^3, one MI355X) + % HBM roofline per storage dtype",
@@ -236,12 +311,15 @@ def main():
                    "pattern": "AB pull, fused collide+stream"},
         "roofline": {"bound": "hbm", "achieved": main_r["achieved_gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": main_r["frac"], "traffic": read_traffic(f"traffic_couette{n}_f64.txt"),
-                     "bytes_per_node": main_r["bytes_per_node"], "copy_kernel_gbs": copy_gbs},
+                     "bytes_per_node": main_r["bytes_per_node"], "copy_kernel_gbs": copy_gbs,
+                     "launch": main_r["launch"]},
         "per_dtype": {k: {"mlups": v["mlups"], "frac": v["frac"], "achieved_gbs": v["achieved_gbs"],
                           "bytes_per_node": v["bytes_per_node"], "ms_per_step": v["ms_per_step"]}
                       for k, v in per.items()},
-        "latency": ({"p50_s": served["p50_s"], "p99_s": served["p99_s"], "workload": "served stream"}
+        "latency": ({"p50_s": served[0]["p50_s"], "p99_s": served[0]["p99_s"],
+                     "workload": f"served stream, Poisson arrivals at {served[0]['rate_hz']} Hz"}
                     if served else None),
+        "porous_channel": porous,
         "batched": batched,
         "served": served,
         "cpu_baseline": cpu,

@@ -21,7 +21,7 @@ STORAGE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2}
 COLLISION = {"bgk": 0, "mrt": 1}
 LAYOUT_NODES, LAYOUT_REF = 0, 1
 BC_KIND = {"pre_bounce_back": 1, "post_bounce_back": 2, "wall_bounce_back": 3, "cavity_z_0": 4, "cavity_z_l": 5,
-           "lid_driven_cavity": 6, "neumann": 7}
+           "lid_driven_cavity": 6, "neumann": 7, "pressure_inlet": 8, "outflow": 9}
 STAGE = {"pre_stream_bcs": 0, "post_stream_bcs": 1, "boundary_conditions": 2}
 PLANE = {"plan_xy_0": 0, "plan_xy_l": 1, "plan_xz_0": 2, "plan_xz_l": 3, "plan_yz_0": 4, "plan_yz_l": 5}
 STEP_WRITE_MACRO = 1
@@ -32,7 +32,8 @@ SYMBOLS = (
     "hlbm_domain_bytes", "hlbm_set_bcs", "hlbm_set_params", "hlbm_init_obst", "hlbm_add_obstacles",
     "hlbm_set_flags", "hlbm_get_flags", "hlbm_set_distribution", "hlbm_set_populations_f64",
     "hlbm_get_populations_f64", "hlbm_step", "hlbm_timestep", "hlbm_synchronize", "hlbm_macro", "hlbm_get_macro",
-    "hlbm_reduce_stats", "hlbm_ghost_update_ref", "hlbm_copy_bench",
+    "hlbm_reduce_stats", "hlbm_ghost_update_ref", "hlbm_copy_bench", "hlbm_autotune", "hlbm_set_launch",
+    "hlbm_set_collision",
     "hlbm_batch_create", "hlbm_batch_destroy", "hlbm_batch_submit", "hlbm_batch_run", "hlbm_batch_status",
     "hlbm_batch_times", "hlbm_batch_result", "hlbm_batch_info_get", "hlbm_clock",
 )
@@ -70,6 +71,19 @@ class Obstacle(ctypes.Structure):
         ("p0", ctypes.c_double * 3),
         ("p1", ctypes.c_double * 3),
         ("radius", ctypes.c_double),
+    ]
+
+
+class TuneInfo(ctypes.Structure):
+    _fields_ = [
+        ("block_x", ctypes.c_int32),
+        ("block_y", ctypes.c_int32),
+        ("npt", ctypes.c_int32),
+        ("xcd_remap", ctypes.c_int32),
+        ("candidates", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
+        ("ms_best", ctypes.c_double),
+        ("ms_default", ctypes.c_double),
     ]
 
 
@@ -118,6 +132,9 @@ def lib():
         "hlbm_reduce_stats": (ctypes.c_int, [P, DP]),
         "hlbm_ghost_update_ref": (ctypes.c_int, [P, DP]),
         "hlbm_copy_bench": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, P, DP]),
+        "hlbm_autotune": (ctypes.c_int, [P, ctypes.c_int32, P, ctypes.POINTER(TuneInfo)]),
+        "hlbm_set_launch": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+        "hlbm_set_collision": (ctypes.c_int, [P, ctypes.c_int32, DP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -55,6 +57,10 @@ struct hlbm_domain {
   double* red_out = nullptr;
   double* tmp = nullptr;  // one FP64 population plane, conversions
   bool any_wall = false;
+  uint32_t* bbm = nullptr;  // bounce-back mask of the bulk kernel, built on demand
+  void* oflow = nullptr;    // outflow values captured before in-place AA steps
+  uint32_t oflow_n = 0;     // nodes per captured plane
+  bool bbm_dirty = true;
   StepParams plan{};
   dim3 block{128, 2, 1};
   int xcd_remap = 1;
@@ -95,17 +101,40 @@ static StepParams base_params(hlbm_domain* h) {
   p.xcd_remap = h->xcd_remap;
   for (int k = 0; k < 10; k++) p.mrt[k] = h->d.mrt_rates[k];
   if (!h->any_wall) p.wall_bb = 0;
+  p.any_wall = h->any_wall ? 1 : 0;
+  p.bbm = h->bbm;
   return p;
+}
+
+// The bulk kernel's bounce-back mask, rebuilt after the flags changed.
+static int ensure_bbm(hlbm_domain* h, hipStream_t s) {
+  if (!h->bbm_dirty && h->bbm) return HLBM_OK;
+  if (!h->bbm) {
+    if (hipMalloc((void**)&h->bbm, (size_t)h->N * 4) != hipSuccess) return fail(HLBM_ERR_NOMEM, "bounce-back mask");
+    h->bytes += (uint64_t)h->N * 4;
+  }
+  const int per[3] = {h->d.periodic[0] ? 1 : 0, h->d.periodic[1] ? 1 : 0, h->d.periodic[2] ? 1 : 0};
+  hipError_t e = launch_build_bbm(h->flags, h->nx, h->ny, h->nz, per, h->bbm, s);
+  if (e != hipSuccess) return fail(HLBM_ERR_HIP, std::string("bounce-back mask: ") + hipGetErrorString(e));
+  h->bbm_dirty = false;
+  return HLBM_OK;
 }
 
 static int launch(hlbm_domain* h, int mode, bool write_macro, hipStream_t s, void* out = nullptr,
                   bool skip_program = false) {
+  if (h->any_wall && h->plan.wall_bb) {
+    const int rc = ensure_bbm(h, s);
+    if (rc) return rc;
+  }
   StepParams p = base_params(h);
   const bool aa = mode_aa(mode);
   p.fin = h->buf[h->cur];
   p.fout = aa ? (out ? out : h->buf[h->cur]) : h->buf[h->cur ^ 1];
   p.write_macro = write_macro ? 1 : 0;
   p.skip_program = skip_program ? 1 : 0;
+  const bool in_place = mode == MODE_AA_EVEN || mode == MODE_AA_ODD;
+  p.oflow = in_place ? h->oflow : nullptr;
+  p.oflow_n = h->oflow_n;
   hipError_t e;
   const int coll = h->d.collision == HLBM_MRT ? COLL_MRT : COLL_BGK;
   if (h->d.storage == HLBM_F64)
@@ -239,7 +268,7 @@ int hlbm_domain_destroy(hlbm_domain* h) {
   if (!h) return HLBM_OK;
   (void)hipDeviceSynchronize();
   for (void* p : {h->buf[0], h->buf[1], (void*)h->flags, (void*)h->m0, (void*)h->m1, (void*)h->red_part,
-                  (void*)h->red_out, (void*)h->tmp})
+                  (void*)h->red_out, (void*)h->tmp, (void*)h->bbm, h->oflow})
     if (p) (void)hipFree(p);
   delete h;
   return HLBM_OK;
@@ -318,8 +347,27 @@ int hlbm_set_bcs(hlbm_domain* h, const hlbm_bc* list, int32_t n) {
           P.prog_planes |= 1 << b.plane;
           P.nprog++;
           break;
+        case HLBM_BC_PRESSURE_INLET:
+        case HLBM_BC_OUTFLOW: {
+          if (b.plane < 0 || b.plane > 5) return fail(HLBM_ERR_ARG, "bad plane");
+          const int dim = plane_dim(b.plane);
+          if (h->d.periodic[dim]) return fail(HLBM_ERR_UNSUPPORTED, "pressure_inlet / outflow need a non-periodic axis");
+          if (b.kind == HLBM_BC_PRESSURE_INLET && !(b.U[0] > 0.0))
+            return fail(HLBM_ERR_ARG, "pressure_inlet density (U[0]) must be > 0");
+          const int n_axis = dim == 0 ? h->nx : (dim == 1 ? h->ny : h->nz);
+          if (b.kind == HLBM_BC_OUTFLOW && n_axis < 3) return fail(HLBM_ERR_UNSUPPORTED, "outflow needs >= 3 nodes");
+          if (P.nprog >= MAX_PROG) return fail(HLBM_ERR_UNSUPPORTED, "too many boundary operators");
+          P.prog[P.nprog].kind = b.kind == HLBM_BC_PRESSURE_INLET ? PK_PRESSURE : PK_OUTFLOW;
+          P.prog[P.nprog].plane = b.plane;
+          for (int d = 0; d < 3; d++) P.prog[P.nprog].U[d] = b.U[d];
+          P.prog_planes |= 1 << b.plane;
+          P.nprog++;
+          break;
+        }
         default:
-          return fail(HLBM_ERR_UNSUPPORTED, "only post_bounce_back, lid_driven_cavity and neumann may run after streaming");
+          return fail(HLBM_ERR_UNSUPPORTED,
+                      "only post_bounce_back, lid_driven_cavity, neumann, pressure_inlet and outflow may run after "
+                      "streaming");
       }
     } else {
       return fail(HLBM_ERR_ARG, "bad stage");
@@ -338,11 +386,41 @@ int hlbm_set_bcs(hlbm_domain* h, const hlbm_bc* list, int32_t n) {
       P.cav[side][j] = 6. * WD(q) * (CX(q) * uxx + CY(q) * uyy + CZ(q) * uzz);
     }
   }
+  // outflow entries carry their ordinal (the slot of their captured values)
+  int n_out = 0;
+  for (int k = 0; k < P.nprog; k++)
+    if (P.prog[k].kind == PK_OUTFLOW) P.prog[k].U[0] = (double)n_out++;
+  // outflow reads the first interior layer as the gather produces it: no earlier
+  // entry may have changed nodes of that layer (entries on planes of another
+  // axis), nor a re-applied bounce-back added a moving-wall term there
+  for (int k = 0; k < P.nprog; k++) {
+    if (P.prog[k].kind != PK_OUTFLOW) continue;
+    for (int j = 0; j < k; j++) {
+      const ProgEntry& e = P.prog[j];
+      if (e.kind == PK_REBB ? (P.cav_on[0] || P.cav_on[1]) : plane_dim(e.plane) != plane_dim(P.prog[k].plane))
+        return fail(HLBM_ERR_UNSUPPORTED, "outflow must precede operators that change its interior layer");
+    }
+  }
   bool has_rebb = false;
   for (int k = 0; k < P.nprog; k++) has_rebb |= P.prog[k].kind == PK_REBB;
   if (has_rebb) {
     for (int dim = 0; dim < 3; dim++)
       if (!h->d.periodic[dim]) P.prog_planes |= dim == 2 ? 3 : (dim == 1 ? 12 : 48);
+  }
+  // in-place AA steps capture the outflow values first (the step overwrites the interior layer)
+  if (h->oflow) {
+    (void)hipFree(h->oflow);
+    h->oflow = nullptr;
+    h->oflow_n = 0;
+  }
+  if (n_out > 0 && h->d.pattern == HLBM_PATTERN_AA) {
+    const uint64_t pl = std::max({(uint64_t)h->nx * h->ny, (uint64_t)h->ny * h->nz, (uint64_t)h->nx * h->nz});
+    const size_t csize = h->d.storage == HLBM_F64 ? 8 : 4;
+    if (hipMalloc(&h->oflow, (size_t)n_out * 5 * pl * csize) != hipSuccess) {
+      h->oflow = nullptr;
+      return fail(HLBM_ERR_NOMEM, "outflow capture buffer");
+    }
+    h->oflow_n = (uint32_t)pl;
   }
   h->plan = P;
   return HLBM_OK;
@@ -353,10 +431,12 @@ int hlbm_init_obst(hlbm_domain* h) {
   HIP_TRY(hipMemset(h->flags, 0, h->N));
   HIP_TRY(hipDeviceSynchronize());
   h->any_wall = false;
+  h->bbm_dirty = true;
   return HLBM_OK;
 }
 
 static int refresh_any_wall(hlbm_domain* h) {
+  h->bbm_dirty = true;
   std::vector<uint8_t> fl(h->N);
   HIP_TRY(hipMemcpy(fl.data(), h->flags, h->N, hipMemcpyDeviceToHost));
   h->any_wall = std::any_of(fl.begin(), fl.end(), [](uint8_t v) { return v != 0; });
@@ -425,6 +505,7 @@ int hlbm_set_flags(hlbm_domain* h, const int32_t* obst, int32_t layout) {
       }
   HIP_TRY(hipMemcpy(h->flags, fl.data(), h->N, hipMemcpyHostToDevice));
   h->any_wall = std::any_of(fl.begin(), fl.end(), [](uint8_t v) { return v != 0; });
+  h->bbm_dirty = true;
   return HLBM_OK;
 }
 
@@ -659,6 +740,156 @@ int hlbm_reduce_stats(hlbm_domain* h, double out[7]) {
   if (!h || !out) return fail(HLBM_ERR_ARG, "NULL argument");
   HIP_TRY(launch_reduce(h->m0, h->m1, h->flags, h->N, h->red_part, h->red_out, 0));
   HIP_TRY(hipMemcpy(out, h->red_out, 7 * 8, hipMemcpyDeviceToHost));
+  return HLBM_OK;
+}
+
+int hlbm_set_collision(hlbm_domain* h, int32_t collision, const double* mrt_rates) {
+  if (!h) return fail(HLBM_ERR_ARG, "NULL domain");
+  if (collision != HLBM_BGK && collision != HLBM_MRT) return fail(HLBM_ERR_ARG, "collision must be BGK or MRT");
+  h->d.collision = collision;
+  h->d.mrt_custom = mrt_rates ? 1 : 0;
+  for (int k = 0; k < 10; k++) h->d.mrt_rates[k] = mrt_rates ? mrt_rates[k] : 0.0;
+  return HLBM_OK;
+}
+
+int hlbm_set_launch(hlbm_domain* h, int32_t block_x, int32_t block_y, int32_t npt, int32_t xcd_remap) {
+  if (!h) return fail(HLBM_ERR_ARG, "NULL domain");
+  if (block_x < 1 || block_y < 1 || block_x * block_y > 256 || (block_x * block_y) % 64 != 0)
+    return fail(HLBM_ERR_ARG, "block_x*block_y must be a multiple of 64 and at most 256");
+  if (npt != 1 && npt != 2) return fail(HLBM_ERR_ARG, "npt must be 1 or 2");
+  h->block = dim3((unsigned)block_x, (unsigned)block_y, 1);
+  h->npt = npt;
+  h->xcd_remap = xcd_remap ? 1 : 0;
+  return HLBM_OK;
+}
+
+namespace {
+struct TuneKey {
+  int nx, ny, nz, storage, collision, pattern, prog_planes, wall_bb, device;
+  bool operator<(const TuneKey& o) const {
+    const int a[9] = {nx, ny, nz, storage, collision, pattern, prog_planes, wall_bb, device};
+    const int b[9] = {o.nx, o.ny, o.nz, o.storage, o.collision, o.pattern, o.prog_planes, o.wall_bb, o.device};
+    return std::lexicographical_compare(a, a + 9, b, b + 9);
+  }
+};
+struct TuneVal {
+  int bx, by, npt, remap;
+  double ms_best, ms_default;
+};
+std::mutex tune_mu;
+std::map<TuneKey, TuneVal> tune_cache;
+}  // namespace
+
+// Time `reps` steps with the domain's current launch shape on scratch memory.
+static int time_shape(hlbm_domain* h, void* scratch, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                      double* ms) {
+  const bool aa = h->d.pattern == HLBM_PATTERN_AA;
+  const int coll = h->d.collision == HLBM_MRT ? COLL_MRT : COLL_BGK;
+  StepParams p = base_params(h);
+  p.write_macro = 0;
+  p.skip_program = 0;
+  auto one = [&](int k) -> hipError_t {
+    int mode;
+    if (aa) {
+      mode = (k & 1) ? MODE_AA_ODD : MODE_AA_EVEN;
+      p.fin = scratch;
+      p.fout = scratch;
+    } else {
+      mode = MODE_FUSED;
+      p.fin = h->buf[h->cur];
+      p.fout = h->buf[h->cur ^ 1];  // overwritten by the next real step anyway
+    }
+    if (h->d.storage == HLBM_F64) return launch_step_f64(coll, mode, h->npt, grid_for(h), h->block, p, s);
+    return launch_step_lowp(h->d.storage, coll, mode, h->npt, grid_for(h), h->block, p, s);
+  };
+  hipError_t e = hipSuccess;
+  for (int k = 0; k < 2 && e == hipSuccess; k++) e = one(k);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(e0, s);
+  for (int k = 0; k < reps && e == hipSuccess; k++) e = one(k);
+  if (e == hipSuccess) e = hipEventRecord(e1, s);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  if (e != hipSuccess) return fail(HLBM_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+  *ms = (double)t / reps;
+  return HLBM_OK;
+}
+
+int hlbm_autotune(hlbm_domain* h, int32_t reps, void* stream, hlbm_tune_info* out) {
+  if (!h) return fail(HLBM_ERR_ARG, "NULL domain");
+  if (reps < 1) reps = 1;
+  hipStream_t s = (hipStream_t)stream;
+  if (h->any_wall && h->plan.wall_bb) {
+    const int rc = ensure_bbm(h, s);
+    if (rc) return rc;
+  }
+  const TuneKey key{h->nx, h->ny, h->nz, h->d.storage, h->d.collision, h->d.pattern, h->plan.prog_planes,
+                    h->any_wall && h->plan.wall_bb ? 1 : 0, h->device};
+  {
+    std::lock_guard<std::mutex> g(tune_mu);
+    auto it = tune_cache.find(key);
+    if (it != tune_cache.end()) {
+      const TuneVal& v = it->second;
+      hlbm_set_launch(h, v.bx, v.by, v.npt, v.remap);
+      if (out) *out = hlbm_tune_info{v.bx, v.by, v.npt, v.remap, 0, 0, v.ms_best, v.ms_default};
+      return HLBM_OK;
+    }
+  }
+  void* scratch = nullptr;
+  if (h->d.pattern == HLBM_PATTERN_AA) {
+    const size_t fbytes = (size_t)h->stride * Q * h->esize;
+    if (hipMalloc(&scratch, fbytes) != hipSuccess) return fail(HLBM_ERR_NOMEM, "autotune scratch populations");
+    if (hipMemcpyAsync(scratch, h->buf[h->cur], fbytes, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+      (void)hipFree(scratch);
+      return fail(HLBM_ERR_HIP, "autotune scratch copy");
+    }
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    if (scratch) (void)hipFree(scratch);
+    return fail(HLBM_ERR_HIP, "autotune events");
+  }
+  const dim3 keep_block = h->block;
+  const int keep_npt = h->npt, keep_remap = h->xcd_remap;
+  static const int shapes[][2] = {{128, 2}, {64, 1}, {64, 2}, {64, 4}, {128, 1}, {256, 1}};
+  TuneVal best{128, 2, 1, 1, 1e300, 0.0};
+  int rc = HLBM_OK, n = 0;
+  for (const auto& sh : shapes) {
+    for (int npt = 1; npt <= 2 && rc == HLBM_OK; npt++) {
+      for (int remap = 1; remap >= 0 && rc == HLBM_OK; remap--) {
+        hlbm_set_launch(h, sh[0], sh[1], npt, remap);
+        double ms = 0.0;
+        rc = time_shape(h, scratch, reps, s, e0, e1, &ms);
+        if (rc) break;
+        n++;
+        if (sh[0] == 128 && sh[1] == 2 && npt == 1 && remap == 1) best.ms_default = ms;
+        if (ms < best.ms_best) {
+          best.bx = sh[0];
+          best.by = sh[1];
+          best.npt = npt;
+          best.remap = remap;
+          best.ms_best = ms;
+        }
+      }
+    }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (scratch) (void)hipFree(scratch);
+  if (rc) {
+    h->block = keep_block;
+    h->npt = keep_npt;
+    h->xcd_remap = keep_remap;
+    return rc;
+  }
+  hlbm_set_launch(h, best.bx, best.by, best.npt, best.remap);
+  {
+    std::lock_guard<std::mutex> g(tune_mu);
+    tune_cache[key] = best;
+  }
+  if (out) *out = hlbm_tune_info{best.bx, best.by, best.npt, best.remap, n, 0, best.ms_best, best.ms_default};
   return HLBM_OK;
 }
 

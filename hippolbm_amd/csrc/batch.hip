@@ -1,9 +1,10 @@
 // batch.hip -- batched domains and the solve server (hlbm_batch_* in include/hlbm.h).
 //
-// Host code.  Solves live in slices of two population arenas (A/B) and one
-// flag arena, carved by a first-fit allocator against an HBM budget.  A chunk
-// is `chunk_steps` launches of batch_kernel (one per time step, captured once
-// as a HIP graph), each walking the tile table of all running solves.
+// Host code.  Solves live in slices of two population arenas (A/B), one flag
+// arena and one bounce-back mask arena, carved by a first-fit allocator against
+// an HBM budget.  A chunk is `chunk_steps` time steps of batch_kernel (one BGK
+// and one MRT launch per step, captured once as a HIP graph), each walking its
+// part of the tile table of all running solves.
 // Admission runs the collide-only kernel on the new slice; retirement runs the
 // stream-only kernel, macro_variables, and copies the result out.
 #include <hip/hip_runtime.h>
@@ -99,6 +100,7 @@ struct hlbm_batch {
   uint64_t arena_elems = 0, flag_bytes = 0;
   void* arena[2] = {nullptr, nullptr};
   uint8_t* flags = nullptr;
+  uint32_t* bbm = nullptr;  // per-node bounce-back masks, same offsets as the flags
   FreeList fl_pop, fl_flag;
   int64_t next_id = 1;
   std::map<int64_t, Solve> solves;
@@ -134,6 +136,8 @@ static StepParams single_params(hlbm_batch* b, const Solve& s, int parity) {
   p.fin = (const char*)b->arena[parity] + s.f_off * b->esize;
   p.fout = (char*)b->arena[parity ^ 1] + s.f_off * b->esize;
   p.flags = b->flags + s.flag_off;
+  p.any_wall = 1;
+  p.bbm = b->bbm + s.flag_off;  // built at admission
   p.wall_bb = s.dom.wall_bb;
   p.plane_bb = 1;
   p.cav_on[0] = s.dom.cav_on[0];
@@ -170,6 +174,7 @@ static hipError_t chunk_launch(hlbm_batch* b, int step) {
   bp.arena[0] = b->arena[0];
   bp.arena[1] = b->arena[1];
   bp.flags = b->flags;
+  bp.bbm = b->bbm;
   bp.step = step;
   if (b->d.storage == HLBM_F64) return launch_batch_f64(b->blocks, bp, b->stream);
   return launch_batch_lowp(b->d.storage, b->blocks, bp, b->stream);
@@ -207,8 +212,8 @@ int hlbm_batch_create(const hlbm_batch_desc* desc, hlbm_batch** out) {
   hlbm_batch* b = new hlbm_batch();
   b->d = *desc;
   b->esize = desc->storage == HLBM_F64 ? 8 : (desc->storage == HLBM_F32 ? 4 : 2);
-  // budget = 2 arenas of 19 planes + flags: per node 38*esize + 1 bytes
-  const uint64_t per_node = 38 * b->esize + 1;
+  // budget = 2 arenas of 19 planes + flags + masks: per node 38*esize + 1 + 4 bytes
+  const uint64_t per_node = 38 * b->esize + 5;
   b->max_nodes = desc->budget_bytes / per_node;
   if (b->max_nodes < 4096) {
     delete b;
@@ -230,16 +235,17 @@ int hlbm_batch_create(const hlbm_batch_desc* desc, hlbm_batch** out) {
   if (hipMalloc(&b->arena[0], b->arena_elems * b->esize) != hipSuccess) return cleanup("arena A");
   if (hipMalloc(&b->arena[1], b->arena_elems * b->esize) != hipSuccess) return cleanup("arena B");
   if (hipMalloc((void**)&b->flags, b->flag_bytes) != hipSuccess) return cleanup("flags");
+  if (hipMalloc((void**)&b->bbm, b->flag_bytes * 4) != hipSuccess) return cleanup("masks");
   if (hipMalloc((void**)&b->d_doms, b->max_doms * sizeof(BatchDomain)) != hipSuccess) return cleanup("domains");
   if (hipMalloc((void**)&b->d_tiles, b->max_tiles * sizeof(BatchTile)) != hipSuccess) return cleanup("tiles");
-  if (hipMalloc((void**)&b->d_ntiles, sizeof(uint32_t)) != hipSuccess) return cleanup("tile count");
+  if (hipMalloc((void**)&b->d_ntiles, 2 * sizeof(uint32_t)) != hipSuccess) return cleanup("tile count");
   if (hipHostMalloc((void**)&b->h_doms, b->max_doms * sizeof(BatchDomain)) != hipSuccess) return cleanup("pinned");
   if (hipHostMalloc((void**)&b->h_tiles, b->max_tiles * sizeof(BatchTile)) != hipSuccess) return cleanup("pinned");
-  if (hipHostMalloc((void**)&b->h_ntiles, sizeof(uint32_t)) != hipSuccess) return cleanup("pinned");
+  if (hipHostMalloc((void**)&b->h_ntiles, 2 * sizeof(uint32_t)) != hipSuccess) return cleanup("pinned");
   if (hipStreamCreate(&b->stream) != hipSuccess) return cleanup("stream");
   if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) return cleanup("events");
-  *b->h_ntiles = 0;
-  if (hipMemcpy(b->d_ntiles, b->h_ntiles, sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+  b->h_ntiles[0] = b->h_ntiles[1] = 0;
+  if (hipMemcpy(b->d_ntiles, b->h_ntiles, 2 * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
     return cleanup("tile count init");
   b->fl_pop.init(b->arena_elems);
   b->fl_flag.init(b->flag_bytes);
@@ -259,8 +265,8 @@ int hlbm_batch_destroy(hlbm_batch* b) {
   if (!b) return HLBM_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
-  for (void* p : {b->arena[0], b->arena[1], (void*)b->flags, (void*)b->d_doms, (void*)b->d_tiles, (void*)b->d_ntiles,
-                  (void*)b->m_tmp, (void*)b->f_tmp})
+  for (void* p : {b->arena[0], b->arena[1], (void*)b->flags, (void*)b->bbm, (void*)b->d_doms, (void*)b->d_tiles,
+                  (void*)b->d_ntiles, (void*)b->m_tmp, (void*)b->f_tmp})
     if (p) (void)hipFree(p);
   for (void* p : {(void*)b->h_doms, (void*)b->h_tiles, (void*)b->h_ntiles})
     if (p) (void)hipHostFree(p);
@@ -283,6 +289,8 @@ int hlbm_batch_submit(hlbm_batch* b, const hlbm_solve_desc* s, int64_t* id) {
   const uint64_t N = (uint64_t)s->nodes[0] * s->nodes[1] * s->nodes[2];
   const uint64_t stride = round_up(N, 128);
   if (stride * 19 > b->arena_elems || N > b->max_nodes) return bfail(HLBM_ERR_NOMEM, "solve can never fit the budget");
+  if ((N - 1) * b->esize > 0xFFFFFFFFull)
+    return bfail(HLBM_ERR_UNSUPPORTED, "solve too large for the batch kernel's 32-bit offsets; use a domain");
   Solve v;
   v.id = b->next_id++;
   v.desc = *s;
@@ -355,6 +363,9 @@ static int admit_one(hlbm_batch* b, Solve& s) {
       if (e != hipSuccess) return bfail(HLBM_ERR_HIP, "rasterize");
     }
   }
+  if (launch_build_bbm(b->flags + s.flag_off, s.dom.nx, s.dom.ny, s.dom.nz, s.dom.per, b->bbm + s.flag_off, st) !=
+      hipSuccess)
+    return bfail(HLBM_ERR_HIP, "bounce-back mask");
   int lo[3] = {0, 0, 0}, hi[3] = {s.dom.nx - 1, s.dom.ny - 1, s.dom.nz - 1};
   void* fa = (char*)b->arena[0] + s.f_off * b->esize;
   if (launch_set_distribution(b->d.storage, fa, s.stride, s.dom.nx, s.dom.ny, lo, hi, s.desc.init_rho, st) != hipSuccess)
@@ -439,16 +450,23 @@ int hlbm_batch_run(hlbm_batch* b, int32_t* n_finished) {
     it = b->queue.erase(it);
     b->tiles_dirty = true;
   }
-  // per-domain chunk lengths and the tile table
+  // per-domain chunk lengths and the tile table (BGK domains' tiles first)
   const int K = b->d.chunk_steps;
-  uint32_t nt = 0;
+  uint32_t nt = 0, nbgk = 0;
   for (size_t k = 0; k < b->running.size(); k++) {
     Solve& s = b->solves[b->running[k]];
     s.dom.active_steps = std::min(K, s.fused_left);
     s.dom.parity = s.parity;
     b->h_doms[k] = s.dom;
-    if (b->tiles_dirty) {
-      for (uint32_t first = 0; first < s.N; first += 256) b->h_tiles[nt++] = BatchTile{(uint32_t)k, first};
+  }
+  if (b->tiles_dirty) {
+    for (int pass = 0; pass < 2; pass++) {
+      for (size_t k = 0; k < b->running.size(); k++) {
+        const Solve& s = b->solves[b->running[k]];
+        if (s.dom.coll != (pass == 0 ? COLL_BGK : COLL_MRT)) continue;
+        for (uint32_t first = 0; first < s.N; first += 256) b->h_tiles[nt++] = BatchTile{(uint32_t)k, first};
+      }
+      if (pass == 0) nbgk = nt;
     }
   }
   hipStream_t st = b->stream;
@@ -456,9 +474,10 @@ int hlbm_batch_run(hlbm_batch* b, int32_t* n_finished) {
     if (hipMemcpyAsync(b->d_doms, b->h_doms, b->running.size() * sizeof(BatchDomain), hipMemcpyHostToDevice, st) != hipSuccess)
       return bfail(HLBM_ERR_HIP, "upload domains");
     if (b->tiles_dirty) {
-      *b->h_ntiles = nt;
+      b->h_ntiles[0] = nbgk;
+      b->h_ntiles[1] = nt;
       if (hipMemcpyAsync(b->d_tiles, b->h_tiles, (size_t)nt * sizeof(BatchTile), hipMemcpyHostToDevice, st) != hipSuccess ||
-          hipMemcpyAsync(b->d_ntiles, b->h_ntiles, sizeof(uint32_t), hipMemcpyHostToDevice, st) != hipSuccess)
+          hipMemcpyAsync(b->d_ntiles, b->h_ntiles, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, st) != hipSuccess)
         return bfail(HLBM_ERR_HIP, "upload tiles");
       b->tiles_dirty = false;
     }
@@ -547,7 +566,7 @@ int hlbm_batch_info_get(hlbm_batch* b, hlbm_batch_info* out) {
   out->running = (int64_t)b->running.size();
   out->done = b->n_done;
   out->chunks = b->chunks;
-  out->bytes_in_use = b->fl_pop.used * b->esize * 2 + b->fl_flag.used;
+  out->bytes_in_use = b->fl_pop.used * b->esize * 2 + b->fl_flag.used * 5;
   out->budget_bytes = b->d.budget_bytes;
   out->node_updates = b->node_updates;
   out->gpu_ms = b->gpu_ms;

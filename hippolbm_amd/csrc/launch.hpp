@@ -9,6 +9,16 @@
 namespace hlbm {
 
 // Blocks of 256 threads covering every node of the program planes.
+// The bulk kernel's fast variant addresses a plane with 32-bit byte offsets
+// and reads obstacle bounce-back from the precomputed per-node mask.  The
+// general variant (64-bit addresses, neighbour flags read directly) runs when
+// some offset, (N-1)*element size, needs more than 32 bits, or when wall
+// bounce-back is active and no mask was supplied.
+inline bool wide_offsets(const StepParams& p, size_t esize) {
+  if (p.any_wall && p.wall_bb && !p.bbm) return true;
+  return (uint64_t)(p.N - 1u) * esize > 0xFFFFFFFFull;
+}
+
 inline unsigned boundary_blocks(const StepParams& p) {
   uint64_t n = 0;
   for (int pl = 0; pl < 6; pl++)

@@ -5,11 +5,18 @@
 
 namespace hlbm {
 
-template <int COLL, int MODE, int NPT>
+template <int COLL, int MODE, int NPT, bool WIDE>
 static hipError_t go(dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
   hipError_t e = hipSuccess;
+  if constexpr (MODE == MODE_AA_EVEN || MODE == MODE_AA_ODD) {
+    if (p.oflow && !p.skip_program) {  // outflow values first: the step below updates nodes in place
+      hipLaunchKernelGGL((outflow_capture_kernel<double, MODE>), dim3((p.oflow_n + 255) / 256), dim3(256), 0, s, p);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+  }
   if (MODE != MODE_AA_FIX) {  // AA_FIX only touches program-plane nodes
-    hipLaunchKernelGGL((step_kernel<double, COLL, MODE, NPT>), grid, block, 0, s, p);
+    hipLaunchKernelGGL((step_kernel<double, COLL, MODE, NPT, WIDE>), grid, block, 0, s, p);
     e = hipGetLastError();
   }
   if (e != hipSuccess || MODE == MODE_COLLIDE || !p.prog_planes) return e;
@@ -19,7 +26,8 @@ static hipError_t go(dim3 grid, dim3 block, const StepParams& p, hipStream_t s) 
 
 template <int COLL, int MODE>
 static hipError_t go_npt(int npt, dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
-  return npt == 2 ? go<COLL, MODE, 2>(grid, block, p, s) : go<COLL, MODE, 1>(grid, block, p, s);
+  if (wide_offsets(p, sizeof(double))) return go<COLL, MODE, 1, true>(grid, block, p, s);
+  return npt == 2 ? go<COLL, MODE, 2, false>(grid, block, p, s) : go<COLL, MODE, 1, false>(grid, block, p, s);
 }
 
 hipError_t launch_step_f64(int coll, int mode, int npt, dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
@@ -41,7 +49,10 @@ hipError_t launch_step_f64(int coll, int mode, int npt, dim3 grid, dim3 block, c
 }
 
 hipError_t launch_batch_f64(unsigned blocks, const BatchParams& bp, hipStream_t s) {
-  hipLaunchKernelGGL((batch_kernel<double>), dim3(blocks), dim3(256), 0, s, bp);
+  hipLaunchKernelGGL((batch_kernel<double, COLL_BGK>), dim3(blocks), dim3(256), 0, s, bp);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((batch_kernel<double, COLL_MRT>), dim3(blocks), dim3(256), 0, s, bp);
   return hipGetLastError();
 }
 

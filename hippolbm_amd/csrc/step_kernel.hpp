@@ -51,7 +51,7 @@ __host__ __device__ constexpr bool mode_aa(int m) { return m >= MODE_AA_EVEN; }
 // in-place AA steps leave wall nodes alone when wall_bounce_back is active:
 // nothing ever reads a wall's populations then
 __host__ __device__ constexpr bool mode_skips_walls(int m) { return m == MODE_AA_EVEN || m == MODE_AA_ODD; }
-enum ProgKind { PK_LID = 1, PK_NEUMANN = 2, PK_REBB = 3 };
+enum ProgKind { PK_LID = 1, PK_NEUMANN = 2, PK_REBB = 3, PK_PRESSURE = 4, PK_OUTFLOW = 5 };
 enum CollKind { COLL_BGK = 0, COLL_MRT = 1 };
 
 constexpr int MAX_PROG = 12;
@@ -71,6 +71,7 @@ struct StepParams {
   const void* fin;
   void* fout;
   const uint8_t* flags; // 0 fluid, 1 wall
+  const uint32_t* bbm;  // per node: bit 0 wall, bit i (1..18) f_i bounces off an upstream wall (wall_bb)
   double* m0;           // rho (written when write_macro)
   double* m1;           // u, SoA 3*N
   int32_t write_macro;
@@ -84,6 +85,9 @@ struct StepParams {
   int32_t gx, gy, gz;   // tile grid of the bulk kernel (tiles of blockDim.x * blockDim.y nodes)
   int32_t xcd_remap;    // map consecutive dispatch slots to the same XCD's run of tiles
   int32_t skip_program; // boundary nodes processed without the post-stream program (AA_EVEN on a clean state)
+  int32_t any_wall;     // some node is a wall
+  int32_t oflow_n;      // nodes per plane slot of oflow
+  void* oflow;          // outflow values captured before an in-place AA step (C type), or null
   double cav[2][5];
   double tau;
   double F[3];
@@ -466,15 +470,30 @@ __device__ __forceinline__ void lid_node(C* f, const double* U) {
 
 // neumann (Zou/He velocity boundary, neumann.hpp:38-258), one face.
 // SHIFT adds the rest-state weight sum (=1) to the density numerator.
-template <class C, bool SHIFT>
-__device__ __forceinline__ void neumann_node(C* f, int plane, const double* U) {
-  const C ux = C(U[0]), uy = C(U[1]), uz = C(U[2]);
+// PRESSURE: the Zou/He density boundary (no counterpart in the reference):
+// rho = rho_in is prescribed, the tangential velocity is zero and the normal
+// velocity follows from the same known-population sum K as the velocity form,
+// rho (1 -+ u_n) = K, i.e. u_n = +-(1 - K / rho_in); the unknown populations
+// then take the velocity form's expressions.
+template <class C, bool SHIFT, bool PRESSURE = false>
+__device__ __forceinline__ void neumann_node(C* f, int plane, const double* U, double rho_in = 1.0) {
+  C ux = PRESSURE ? C(0) : C(U[0]), uy = PRESSURE ? C(0) : C(U[1]), uz = PRESSURE ? C(0) : C(U[2]);
   const C base = SHIFT ? C(1) : C(0);
   const C h = C(1. / 2.), t = C(1. / 3.), s = C(1. / 6.);
+  const C rin = C(rho_in);
+  // rho from the known sum K over (1 + un) (high face) or (1 - un) (low face)
+  auto density = [&](C K, C& un, bool high) -> C {
+    if constexpr (PRESSURE) {
+      un = high ? K / rin - C(1.) : C(1.) - K / rin;
+      return rin;
+    } else {
+      return high ? K / (C(1.) + un) : K / (C(1.) - un);
+    }
+  };
   switch (plane) {
     case 5: {  // plan_yz_l
-      const C rho = ((f[3] + f[4] + f[5] + f[6] + f[15] + f[17] + f[18] + f[16] + f[0] +
-                      C(2.) * (f[1] + f[7] + f[9] + f[11] + f[13])) + base) / (C(1.) + ux);
+      const C rho = density((f[3] + f[4] + f[5] + f[6] + f[15] + f[17] + f[18] + f[16] + f[0] +
+                             C(2.) * (f[1] + f[7] + f[9] + f[11] + f[13])) + base, ux, true);
       const C nyx = h * (f[3] + f[15] + f[17] - (f[4] + f[18] + f[16])) - t * rho * uy;
       const C nzx = h * (f[5] + f[18] + f[15] - (f[6] + f[17] + f[16])) - t * rho * uz;
       f[2] = f[1] - t * rho * ux;
@@ -485,8 +504,8 @@ __device__ __forceinline__ void neumann_node(C* f, int plane, const double* U) {
       break;
     }
     case 4: {  // plan_yz_0
-      const C rho = ((f[3] + f[4] + f[5] + f[6] + f[15] + f[17] + f[18] + f[16] + f[0] +
-                      C(2.) * (f[2] + f[10] + f[8] + f[14] + f[12])) + base) / (C(1.) - ux);
+      const C rho = density((f[3] + f[4] + f[5] + f[6] + f[15] + f[17] + f[18] + f[16] + f[0] +
+                             C(2.) * (f[2] + f[10] + f[8] + f[14] + f[12])) + base, ux, false);
       const C nyx = h * (f[3] + f[15] + f[17] - (f[4] + f[18] + f[16])) - t * rho * uy;
       const C nzx = h * (f[5] + f[18] + f[15] - (f[6] + f[17] + f[16])) - t * rho * uz;
       f[1] = f[2] + t * rho * ux;
@@ -497,8 +516,8 @@ __device__ __forceinline__ void neumann_node(C* f, int plane, const double* U) {
       break;
     }
     case 3: {  // plan_xz_l
-      const C rho = ((f[1] + f[2] + f[5] + f[6] + f[11] + f[13] + f[14] + f[12] + f[0] +
-                      C(2.) * (f[3] + f[7] + f[10] + f[15] + f[17])) + base) / (C(1.) + uy);
+      const C rho = density((f[1] + f[2] + f[5] + f[6] + f[11] + f[13] + f[14] + f[12] + f[0] +
+                             C(2.) * (f[3] + f[7] + f[10] + f[15] + f[17])) + base, uy, true);
       const C nxy = h * (f[1] + f[11] + f[13] - (f[2] + f[14] + f[12])) - t * rho * ux;
       const C nzy = h * (f[5] + f[11] + f[14] - (f[6] + f[13] + f[12])) - t * rho * uz;
       f[4] = f[3] - t * rho * uy;
@@ -509,8 +528,8 @@ __device__ __forceinline__ void neumann_node(C* f, int plane, const double* U) {
       break;
     }
     case 2: {  // plan_xz_0
-      const C rho = ((f[1] + f[2] + f[5] + f[6] + f[11] + f[13] + f[14] + f[12] + f[0] +
-                      C(2.) * (f[4] + f[9] + f[8] + f[18] + f[16])) + base) / (C(1.) - uy);
+      const C rho = density((f[1] + f[2] + f[5] + f[6] + f[11] + f[13] + f[14] + f[12] + f[0] +
+                             C(2.) * (f[4] + f[9] + f[8] + f[18] + f[16])) + base, uy, false);
       const C nxy = h * (f[1] + f[11] + f[13] - (f[2] + f[14] + f[12])) - t * rho * ux;
       const C nzy = h * (f[5] + f[11] + f[14] - (f[6] + f[13] + f[12])) - t * rho * uz;
       f[3] = f[4] + t * rho * uy;
@@ -521,8 +540,8 @@ __device__ __forceinline__ void neumann_node(C* f, int plane, const double* U) {
       break;
     }
     case 1: {  // plan_xy_l
-      const C rho = ((f[0] + f[1] + f[2] + f[3] + f[4] + f[7] + f[9] + f[10] + f[8] +
-                      C(2.) * (f[5] + f[11] + f[14] + f[15] + f[18])) + base) / (C(1.) + uz);
+      const C rho = density((f[0] + f[1] + f[2] + f[3] + f[4] + f[7] + f[9] + f[10] + f[8] +
+                             C(2.) * (f[5] + f[11] + f[14] + f[15] + f[18])) + base, uz, true);
       const C nxz = h * (f[1] + f[7] + f[9] - (f[2] + f[10] + f[8])) - t * rho * ux;
       const C nyz = h * (f[3] + f[7] + f[10] - (f[4] + f[9] + f[8])) - t * rho * uy;
       f[6] = f[5] - t * rho * uz;
@@ -533,8 +552,8 @@ __device__ __forceinline__ void neumann_node(C* f, int plane, const double* U) {
       break;
     }
     default: {  // 0: plan_xy_0
-      const C rho = ((f[0] + f[1] + f[2] + f[3] + f[4] + f[7] + f[9] + f[10] + f[8] +
-                      C(2.) * (f[6] + f[13] + f[12] + f[17] + f[16])) + base) / (C(1.) - uz);
+      const C rho = density((f[0] + f[1] + f[2] + f[3] + f[4] + f[7] + f[9] + f[10] + f[8] +
+                             C(2.) * (f[6] + f[13] + f[12] + f[17] + f[16])) + base, uz, false);
       const C nxz = h * (f[1] + f[7] + f[9] - (f[2] + f[10] + f[8])) - t * rho * ux;
       const C nyz = h * (f[3] + f[7] + f[10] - (f[4] + f[9] + f[8])) - t * rho * uy;
       f[5] = f[6] + t * rho * uz;
@@ -551,60 +570,69 @@ __device__ __forceinline__ void neumann_node(C* f, int plane, const double* U) {
 // the kernel
 // ---------------------------------------------------------------------------
 // Plane bits of a node: bit p set when the node lies on plane p (hlbm_plane order).
+// Index of node (x, y, z) inside plane pl: a + b*na, a the fastest in-plane
+// axis (the order boundary_kernel walks a plane in).
+__device__ __forceinline__ uint32_t plane_slot(int pl, int x, int y, int z, int nx, int ny) {
+  const int dim = plane_dim(pl);
+  const uint32_t na = dim == 0 ? (uint32_t)ny : (uint32_t)nx;
+  const uint32_t a = dim == 0 ? (uint32_t)y : (uint32_t)x;
+  const uint32_t b = dim == 2 ? (uint32_t)y : (uint32_t)z;
+  return b * na + a;
+}
+
 __device__ __forceinline__ int node_planes(int x, int y, int z, int nx, int ny, int nz) {
   return (z == 0 ? 1 : 0) | (z == nz - 1 ? 2 : 0) | (y == 0 ? 4 : 0) | (y == ny - 1 ? 8 : 0) | (x == 0 ? 16 : 0) |
          (x == nx - 1 ? 32 : 0);
 }
 
-// Full per-node pipeline.  PROG selects whether the post-stream boundary
-// program is compiled in: the bulk kernel never runs it (its nodes are not on
-// a program plane), the boundary kernel always may.
 // Phase 1: the 19 populations arriving at the node (pull + bounce-back), or
 // the node's own populations when the state is pre-collision (MODE_COLLIDE).
-template <class S, int MODE>
-__device__ __forceinline__ void gather_node(const StepParams& p, int x, int y, int z, uint32_t idx, bool wall,
-                                            typename StoreOps<S>::C* f) {
+//
+// pull_one: the post-stream value of direction I at node (x, y, z).
+// A/B: f*_i(s) sits in plane i at s and f*_opp(i)(x) in plane opp(i) at x.
+// AA (swapped state): f*_i(s) sits in plane opp(i) at s, f*_opp(i)(x) in plane i at x.
+// Modes that read the node's own populations return its natural slot.
+template <class S, int MODE, int I>
+__device__ __forceinline__ typename StoreOps<S>::C pull_one(const StepParams& p, int x, int y, int z, uint32_t idx,
+                                                            bool wall) {
   using O = StoreOps<S>;
   using C = typename O::C;
-  const int nx = p.nx, ny = p.ny, nz = p.nz;
   const S* __restrict__ fin = (const S*)p.fin;
   const size_t st = p.stride;
   if constexpr (mode_reads_own(MODE)) {
-#pragma unroll
-    for (int i = 0; i < Q; i++) f[i] = O::ld(fin, (size_t)i * st + idx);
+    return O::ld(fin, (size_t)I * st + idx);
   } else {
-    // A/B: f*_i(s) sits in plane i at s and f*_opp(i)(x) in plane opp(i) at x.
-    // AA (swapped state): f*_i(s) sits in plane opp(i) at s, f*_opp(i)(x) in plane i at x.
     constexpr bool AA = mode_aa(MODE);
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
     // upstream coordinates; -1 marks a non-periodic face (bounce-back)
-    const int xm = x > 0 ? x - 1 : (p.per[0] ? nx - 1 : -1);
-    const int xp = x < nx - 1 ? x + 1 : (p.per[0] ? 0 : -1);
-    const int ym = y > 0 ? y - 1 : (p.per[1] ? ny - 1 : -1);
-    const int yp = y < ny - 1 ? y + 1 : (p.per[1] ? 0 : -1);
-    const int zm = z > 0 ? z - 1 : (p.per[2] ? nz - 1 : -1);
-    const int zp = z < nz - 1 ? z + 1 : (p.per[2] ? 0 : -1);
-    const bool wbb = p.wall_bb && !wall;
-    static_for([&](auto ic) {
-      constexpr int i = decltype(ic)::value;
-      const int sx = CX(i) == 0 ? x : (CX(i) == 1 ? xm : xp);
-      const int sy = CY(i) == 0 ? y : (CY(i) == 1 ? ym : yp);
-      const int sz = CZ(i) == 0 ? z : (CZ(i) == 1 ? zm : zp);
-      if (sx < 0 || sy < 0 || sz < 0) {
-        C v = O::ld(fin, (size_t)(AA ? i : OPP(i)) * st + idx);
-        if constexpr (CZ(i) != 0) {
-          constexpr int side = CZ(i) == 1 ? 0 : 1;
-          if (sz < 0 && p.cav_gather && p.cav_on[side]) v += C(p.cav[side][cavity_slot(i)]);
-        }
-        f[i] = v;
-      } else {
-        const uint32_t sidx = ((uint32_t)sz * (uint32_t)ny + (uint32_t)sy) * (uint32_t)nx + (uint32_t)sx;
-        if (i != 0 && wbb && p.flags[sidx] != 0)
-          f[i] = O::ld(fin, (size_t)(AA ? i : OPP(i)) * st + idx);
-        else
-          f[i] = O::ld(fin, (size_t)(AA ? OPP(i) : i) * st + sidx);
+    int sx = x, sy = y, sz = z;
+    if constexpr (CX(I) == 1) sx = x > 0 ? x - 1 : (p.per[0] ? nx - 1 : -1);
+    if constexpr (CX(I) == -1) sx = x < nx - 1 ? x + 1 : (p.per[0] ? 0 : -1);
+    if constexpr (CY(I) == 1) sy = y > 0 ? y - 1 : (p.per[1] ? ny - 1 : -1);
+    if constexpr (CY(I) == -1) sy = y < ny - 1 ? y + 1 : (p.per[1] ? 0 : -1);
+    if constexpr (CZ(I) == 1) sz = z > 0 ? z - 1 : (p.per[2] ? nz - 1 : -1);
+    if constexpr (CZ(I) == -1) sz = z < nz - 1 ? z + 1 : (p.per[2] ? 0 : -1);
+    if (sx < 0 || sy < 0 || sz < 0) {
+      C v = O::ld(fin, (size_t)(AA ? I : OPP(I)) * st + idx);
+      if constexpr (CZ(I) != 0) {
+        constexpr int side = CZ(I) == 1 ? 0 : 1;
+        if (sz < 0 && p.cav_gather && p.cav_on[side]) v += C(p.cav[side][cavity_slot(I)]);
       }
-    });
+      return v;
+    }
+    const uint32_t sidx = ((uint32_t)sz * (uint32_t)ny + (uint32_t)sy) * (uint32_t)nx + (uint32_t)sx;
+    if (I != 0 && p.wall_bb && !wall && p.flags[sidx] != 0) return O::ld(fin, (size_t)(AA ? I : OPP(I)) * st + idx);
+    return O::ld(fin, (size_t)(AA ? OPP(I) : I) * st + sidx);
   }
+}
+
+template <class S, int MODE>
+__device__ __forceinline__ void gather_node(const StepParams& p, int x, int y, int z, uint32_t idx, bool wall,
+                                            typename StoreOps<S>::C* f) {
+  static_for([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    f[i] = pull_one<S, MODE, i>(p, x, y, z, idx, wall);
+  });
 }
 
 // Phase 2: the post-stream boundary program (post_bounce_back re-application,
@@ -640,12 +668,45 @@ __device__ __forceinline__ void apply_program(const StepParams& p, int x, int y,
                 });
               }
             });
-          } else if ((bmask >> e.plane) & 1) {
-            if (!wall) {
-              if (e.kind == PK_LID)
-                lid_node<C, SHIFT>(f, e.U);
-              else
-                neumann_node<C, SHIFT>(f, e.plane, e.U);
+          } else if (((bmask >> e.plane) & 1) && !wall) {
+            if (e.kind == PK_LID) {
+              lid_node<C, SHIFT>(f, e.U);
+            } else if (e.kind == PK_NEUMANN) {
+              neumann_node<C, SHIFT>(f, e.plane, e.U);
+            } else if (e.kind == PK_PRESSURE) {
+              neumann_node<C, SHIFT, true>(f, e.plane, e.U, e.U[0]);
+            } else {
+              // outflow (no counterpart in the reference): the populations entering
+              // from outside take their post-stream values at the first interior node
+              static_for<0, 6>([&](auto pc) {
+                constexpr int pl = decltype(pc)::value;
+                constexpr int dim = pl < 2 ? 2 : (pl < 4 ? 1 : 0);
+                constexpr int inward = (pl & 1) ? -1 : 1;
+                if (e.plane == pl) {
+                  if (p.oflow) {
+                    // in-place AA step: the values were captured before any node was updated
+                    const C* cap = (const C*)p.oflow + (size_t)e.U[0] * 5 * (size_t)p.oflow_n +
+                                   plane_slot(pl, x, y, z, nx, ny);
+                    int j = 0;
+                    static_for<1, Q>([&](auto ic) {
+                      constexpr int i = decltype(ic)::value;
+                      constexpr int cd = dim == 0 ? CX(i) : (dim == 1 ? CY(i) : CZ(i));
+                      if constexpr (cd == inward) f[i] = cap[(size_t)(j++) * p.oflow_n];
+                    });
+                  } else {
+                    const int ax = x + (dim == 0 ? inward : 0), ay = y + (dim == 1 ? inward : 0),
+                              az = z + (dim == 2 ? inward : 0);
+                    const uint32_t aidx =
+                        ((uint32_t)az * (uint32_t)ny + (uint32_t)ay) * (uint32_t)nx + (uint32_t)ax;
+                    const bool awall = p.flags[aidx] != 0;
+                    static_for<1, Q>([&](auto ic) {
+                      constexpr int i = decltype(ic)::value;
+                      constexpr int cd = dim == 0 ? CX(i) : (dim == 1 ? CY(i) : CZ(i));
+                      if constexpr (cd == inward) f[i] = pull_one<S, MODE, i>(p, ax, ay, az, aidx, awall);
+                    });
+                  }
+                }
+              });
             }
           }
         }
@@ -653,15 +714,12 @@ __device__ __forceinline__ void apply_program(const StepParams& p, int x, int y,
     }
 }
 
-// Phase 3: macro_variables (optionally written out), collision, store.
+// Phase 3: macro_variables (optionally written out) and collision.
 template <class S, int COLL, int MODE>
-__device__ __forceinline__ void finish_node(const StepParams& p, int x, int y, int z, uint32_t idx, bool wall,
-                                            typename StoreOps<S>::C* f) {
+__device__ __forceinline__ void collide_node(const StepParams& p, uint32_t idx, bool wall,
+                                             typename StoreOps<S>::C* f) {
   using O = StoreOps<S>;
-  using C = typename O::C;
   constexpr bool SHIFT = O::shifted;
-  S* __restrict__ fout = (S*)p.fout;
-  const size_t st = p.stride;
   if constexpr (mode_collides(MODE)) {
     if constexpr (!SHIFT) {
       double rho, ux, uy, uz;
@@ -720,7 +778,16 @@ __device__ __forceinline__ void finish_node(const StepParams& p, int x, int y, i
       }
     }
   }
+}
 
+// Phase 4: store (64-bit per-lane addressing; any node).
+template <class S, int MODE>
+__device__ __forceinline__ void store_node(const StepParams& p, int x, int y, int z, uint32_t idx, bool wall,
+                                           typename StoreOps<S>::C* f) {
+  using O = StoreOps<S>;
+  using C = typename O::C;
+  S* __restrict__ fout = (S*)p.fout;
+  const size_t st = p.stride;
   if constexpr (MODE == MODE_AA_EVEN) {
 #pragma unroll
     for (int i = 0; i < Q; i++) O::st(fout, (size_t)OPP(i) * st + idx, f[i]);
@@ -762,6 +829,194 @@ __device__ __forceinline__ void finish_node(const StepParams& p, int x, int y, i
   }
 }
 
+template <class S, int COLL, int MODE>
+__device__ __forceinline__ void finish_node(const StepParams& p, int x, int y, int z, uint32_t idx, bool wall,
+                                            typename StoreOps<S>::C* f) {
+  collide_node<S, COLL, MODE>(p, idx, wall, f);
+  store_node<S, MODE>(p, x, y, z, idx, wall, f);
+}
+
+// ---------------------------------------------------------------------------
+// bulk-node addressing
+// ---------------------------------------------------------------------------
+// In the bulk kernel every population access is a wave-uniform 64-bit plane
+// base (scalar registers) plus a per-lane 32-bit byte offset: one vector add
+// per direction instead of a 64-bit address per lane.  Valid while every byte
+// offset of a plane, (N-1)*sizeof(S), fits in 32 bits (the host picks the
+// 64-bit variant of the bulk kernel otherwise).
+template <class T>
+__device__ __forceinline__ T* at_bytes(T* base, uint32_t off) {
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return (T*)((B*)base + off);
+}
+// Plane bases are walked with one running pointer held in scalar registers and
+// opaque to the optimizer: otherwise it reassociates base + i*stride + offset
+// into a per-lane 64-bit address and keeps 19 products i*stride live.
+template <class T>
+__device__ __forceinline__ T* scalar_ptr(T* p) {
+  auto g = (__attribute__((address_space(1))) T*)p;  // global memory: keeps global_* (not flat_*) accesses
+  asm("" : "+s"(g));
+  return (T*)g;
+}
+// A per-lane byte offset defined anew where it is used, so the selector sees
+// (scalar base + zero-extended 32-bit offset) in the same block and emits the
+// scalar-base form of the access.
+__device__ __forceinline__ uint32_t lane_offset(uint32_t o) {
+  asm("" : "+v"(o));
+  return o;
+}
+
+// Byte offsets from a node to its two neighbours on each axis, with the
+// periodic wrap applied (arithmetic modulo 2^32).  A neighbour outside a
+// non-periodic face gets offset 0 (the node itself: a valid address whose value
+// the bounce-back replaces) and sets its face bit.
+struct NodeNbrs {
+  uint32_t dm[3];  // to the node at -1 on the axis
+  uint32_t dp[3];  // to the node at +1 on the axis
+  bool fm[3];      // -1 neighbour is outside a non-periodic face
+  bool fp[3];      // +1 neighbour is outside a non-periodic face
+};
+
+template <class S>
+__device__ __forceinline__ NodeNbrs node_nbrs(const StepParams& p, int x, int y, int z) {
+  NodeNbrs a;
+  const int c[3] = {x, y, z};
+  const int n[3] = {p.nx, p.ny, p.nz};
+  const uint32_t unit[3] = {(uint32_t)sizeof(S), (uint32_t)sizeof(S) * (uint32_t)p.nx,
+                            (uint32_t)sizeof(S) * (uint32_t)p.nx * (uint32_t)p.ny};
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const bool lo = c[d] == 0, hi = c[d] == n[d] - 1;
+    const uint32_t span = (uint32_t)(n[d] - 1) * unit[d];
+    a.dm[d] = lo ? (p.per[d] ? span : 0u) : 0u - unit[d];
+    a.dp[d] = hi ? (p.per[d] ? 0u - span : 0u) : unit[d];
+    a.fm[d] = lo && !p.per[d];
+    a.fp[d] = hi && !p.per[d];
+  }
+  return a;
+}
+
+// offset of x - c_i (UP) or x + c_i (!UP) relative to x
+template <int I, bool UP>
+__device__ __forceinline__ uint32_t nbr_offset(const NodeNbrs& a) {
+  uint32_t o = 0;
+  if constexpr (CX(I) != 0) o += (CX(I) == 1) == UP ? a.dm[0] : a.dp[0];
+  if constexpr (CY(I) != 0) o += (CY(I) == 1) == UP ? a.dm[1] : a.dp[1];
+  if constexpr (CZ(I) != 0) o += (CZ(I) == 1) == UP ? a.dm[2] : a.dp[2];
+  return o;
+}
+// x - c_i (UP) or x + c_i (!UP) lies outside a non-periodic face; zface: outside a z face
+template <int I, bool UP>
+__device__ __forceinline__ bool nbr_outside(const NodeNbrs& a, bool& zface) {
+  bool o = false;
+  zface = false;
+  if constexpr (CX(I) != 0) o = o || ((CX(I) == 1) == UP ? a.fm[0] : a.fp[0]);
+  if constexpr (CY(I) != 0) o = o || ((CY(I) == 1) == UP ? a.fm[1] : a.fp[1]);
+  if constexpr (CZ(I) != 0) {
+    zface = (CZ(I) == 1) == UP ? a.fm[2] : a.fp[2];
+    o = o || zface;
+  }
+  return o;
+}
+
+// Gather for a bulk node (same result as gather_node): all 19 pulls are issued
+// with wrapped offsets, then, only in waves holding a bounce-back lane, the
+// lanes whose upstream node is outside a non-periodic face or (wall_bb) a wall
+// take f*_opp(i)(x) instead, plus the moving-wall term on a cavity face.
+template <class S, int MODE>
+__device__ __forceinline__ void gather_bulk(const StepParams& p, uint32_t bo, const NodeNbrs& a, uint32_t wbits,
+                                            typename StoreOps<S>::C* f) {
+  using O = StoreOps<S>;
+  using C = typename O::C;
+  const S* __restrict__ fin = (const S*)p.fin;
+  const size_t st = p.stride;
+  const S* b = fin;
+  bo = lane_offset(bo);
+  if constexpr (mode_reads_own(MODE)) {
+    static_for([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      b = scalar_ptr(j ? b + st : b);
+      f[j] = O::ld(at_bytes(b, bo), 0);
+    });
+  } else {
+    constexpr bool AA = mode_aa(MODE);
+    // plane j holds direction i = j (A/B) or i = opp(j) (AA, swapped)
+    static_for([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      constexpr int i = AA ? OPP(j) : j;
+      b = scalar_ptr(j ? b + st : b);
+      f[i] = O::ld(at_bytes(b, bo + nbr_offset<i, true>(a)), 0);
+    });
+    const bool edge = a.fm[0] || a.fp[0] || a.fm[1] || a.fp[1] || a.fm[2] || a.fp[2] || wbits > 1u;
+    if (__any(edge)) {
+      static_for<1, Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        bool zf;
+        const bool out = nbr_outside<i, true>(a, zf);
+        if (out || ((wbits >> i) & 1u)) {
+          C v = O::ld(at_bytes(fin + (size_t)(AA ? i : OPP(i)) * st, bo), 0);
+          if constexpr (CZ(i) != 0) {
+            constexpr int side = CZ(i) == 1 ? 0 : 1;
+            if (zf && p.cav_gather && p.cav_on[side]) v += C(p.cav[side][cavity_slot(i)]);
+          }
+          f[i] = v;
+        }
+      });
+    }
+  }
+}
+
+// Store for a bulk node (same result as store_node).
+template <class S, int MODE>
+__device__ __forceinline__ void store_bulk(const StepParams& p, uint32_t bo, const NodeNbrs& a, uint32_t wbits,
+                                           typename StoreOps<S>::C* f) {
+  using O = StoreOps<S>;
+  using C = typename O::C;
+  S* __restrict__ fout = (S*)p.fout;
+  const size_t st = p.stride;
+  S* b = fout;
+  bo = lane_offset(bo);
+  if constexpr (MODE == MODE_AA_EVEN) {
+    static_for([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      b = scalar_ptr(j ? b + st : b);
+      O::st(at_bytes(b, bo), 0, f[OPP(j)]);
+    });
+  } else if constexpr (MODE == MODE_AA_ODD) {
+    // push to x + c_i; a face or (wall_bb) wall downstream bounces into slot opp(i) of x
+    const bool edge = a.fm[0] || a.fp[0] || a.fm[1] || a.fp[1] || a.fm[2] || a.fp[2] || wbits > 1u;
+    if (!__any(edge)) {
+      static_for([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        b = scalar_ptr(i ? b + st : b);
+        O::st(at_bytes(b, bo + nbr_offset<i, false>(a)), 0, f[i]);
+      });
+    } else {
+      static_for([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        bool zf;
+        const bool out = nbr_outside<i, false>(a, zf);
+        if (i != 0 && (out || ((wbits >> OPP(i)) & 1u))) {
+          C v = f[i];
+          if constexpr (CZ(i) != 0) {
+            constexpr int side = CZ(i) == -1 ? 0 : 1;  // face of the incoming direction opp(i)
+            if (zf && p.cav_gather && p.cav_on[side]) v += C(p.cav[side][cavity_slot(OPP(i))]);
+          }
+          O::st(at_bytes(fout + (size_t)OPP(i) * st, bo), 0, v);
+        } else {
+          O::st(at_bytes(fout + (size_t)i * st, bo + nbr_offset<i, false>(a)), 0, f[i]);
+        }
+      });
+    }
+  } else {
+    static_for([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      b = scalar_ptr(j ? b + st : b);
+      O::st(at_bytes(b, bo), 0, f[j]);
+    });
+  }
+}
+
 // Full per-node pipeline.  PROG selects whether the post-stream boundary
 // program is compiled in: the bulk kernel never runs it (its nodes are not on
 // a program plane), the boundary kernel always may.
@@ -783,7 +1038,7 @@ __device__ __forceinline__ void process_node(const StepParams& p, int x, int y, 
 // A tile is blockDim.x nodes along x by blockDim.y*NPT rows along y; each
 // thread gathers its NPT nodes first (19*NPT loads in flight), then collides
 // and stores them.
-template <class S, int COLL, int MODE, int NPT>
+template <class S, int COLL, int MODE, int NPT, bool WIDE>
 __global__ void __launch_bounds__(256) step_kernel(const StepParams p) {
   using C = typename StoreOps<S>::C;
   // 1-D dispatch over tiles.  Workgroups are handed to the 8 XCDs round-robin;
@@ -801,6 +1056,41 @@ __global__ void __launch_bounds__(256) step_kernel(const StepParams p) {
   const int z = (int)(r / (uint32_t)p.gy);
   const int x = bx * blockDim.x + threadIdx.x;
   if (x >= p.nx || z >= p.nz) return;
+  if constexpr (!WIDE) {
+    C f[NPT][Q];
+    uint32_t bo[NPT], wbits[NPT];
+    NodeNbrs a[NPT];
+    bool wall[NPT], act[NPT];
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+      const int y = (by * NPT + k) * blockDim.y + threadIdx.y;
+      act[k] = y < p.ny && !(MODE != MODE_COLLIDE && (node_planes(x, y, z, p.nx, p.ny, p.nz) & p.prog_planes));
+      const uint32_t idx = ((uint32_t)z * (uint32_t)p.ny + (uint32_t)y) * (uint32_t)p.nx + (uint32_t)x;
+      bo[k] = idx * (uint32_t)sizeof(S);
+      wbits[k] = 0u;
+      wall[k] = false;
+      if (act[k] && p.any_wall) {
+        if (p.wall_bb && p.bbm) {  // the host never launches this variant with wall_bb and no mask
+          wbits[k] = p.bbm[idx];
+          wall[k] = (wbits[k] & 1u) != 0u;
+        } else {
+          wall[k] = p.flags[idx] != 0;
+        }
+        if (mode_skips_walls(MODE) && wall[k] && p.wall_bb) act[k] = false;
+      }
+      if (act[k]) {
+        a[k] = node_nbrs<S>(p, x, y, z);
+        gather_bulk<S, MODE>(p, bo[k], a[k], wbits[k], f[k]);
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NPT; k++) {
+      if (act[k]) {
+        collide_node<S, COLL, MODE>(p, bo[k] / (uint32_t)sizeof(S), wall[k], f[k]);
+        store_bulk<S, MODE>(p, bo[k], a[k], wbits[k], f[k]);
+      }
+    }
+  } else {
   C f[NPT][Q];
   uint32_t idx[NPT];
   bool wall[NPT], act[NPT];
@@ -818,6 +1108,7 @@ __global__ void __launch_bounds__(256) step_kernel(const StepParams p) {
 #pragma unroll
   for (int k = 0; k < NPT; k++)
     if (act[k]) finish_node<S, COLL, MODE>(p, x, (by * NPT + k) * blockDim.y + threadIdx.y, z, idx[k], wall[k], f[k]);
+  }
 }
 
 // Boundary kernel: the nodes of the program planes, each once (a node on
@@ -843,6 +1134,46 @@ __global__ void __launch_bounds__(256) boundary_kernel(const StepParams p) {
       return;
     }
     t -= cnt;
+  }
+}
+
+// Outflow capture for an in-place AA step: before any node of the step is
+// updated, each outflow face node's incoming populations are evaluated at its
+// first interior node (as the outflow entry would) and stored in p.oflow, slot
+// ((ordinal * 5 + j) * oflow_n + plane_slot).  Outflow entries carry their
+// ordinal among the outflow entries in U[0].
+template <class S, int MODE>
+__global__ void __launch_bounds__(256) outflow_capture_kernel(const StepParams p) {
+  using C = typename StoreOps<S>::C;
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = p.nx, ny = p.ny, nz = p.nz;
+  for (int k = 0; k < p.nprog; k++) {
+    const ProgEntry& e = p.prog[k];
+    if (e.kind != PK_OUTFLOW) continue;
+    static_for<0, 6>([&](auto pc) {
+      constexpr int pl = decltype(pc)::value;
+      constexpr int dim = pl < 2 ? 2 : (pl < 4 ? 1 : 0);
+      constexpr int inward = (pl & 1) ? -1 : 1;
+      if (e.plane != pl) return;
+      const uint32_t na = dim == 0 ? (uint32_t)ny : (uint32_t)nx;
+      const uint32_t nb = dim == 2 ? (uint32_t)ny : (uint32_t)nz;
+      if (t >= na * nb) return;
+      const int a = (int)(t % na), b = (int)(t / na);
+      const int c = (pl & 1) ? (dim == 0 ? nx - 1 : (dim == 1 ? ny - 1 : nz - 1)) : 0;
+      const int x = dim == 0 ? c : a;
+      const int y = dim == 0 ? a : (dim == 1 ? c : b);
+      const int z = dim == 2 ? c : b;
+      const int ax = x + (dim == 0 ? inward : 0), ay = y + (dim == 1 ? inward : 0), az = z + (dim == 2 ? inward : 0);
+      const uint32_t aidx = ((uint32_t)az * (uint32_t)ny + (uint32_t)ay) * (uint32_t)nx + (uint32_t)ax;
+      const bool awall = p.flags[aidx] != 0;
+      C* cap = (C*)p.oflow + (size_t)e.U[0] * 5 * (size_t)p.oflow_n + t;
+      int j = 0;
+      static_for<1, Q>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        constexpr int cd = dim == 0 ? CX(i) : (dim == 1 ? CY(i) : CZ(i));
+        if constexpr (cd == inward) cap[(size_t)(j++) * p.oflow_n] = pull_one<S, MODE, i>(p, ax, ay, az, aidx, awall);
+      });
+    });
   }
 }
 
@@ -886,9 +1217,10 @@ struct BatchTile {
 struct BatchParams {
   const BatchDomain* doms;
   const BatchTile* tiles;
-  const uint32_t* ntiles;  // device-resident count (graphs stay valid as the mix changes)
+  const uint32_t* ntiles;  // device-resident counts: [BGK tiles, all tiles]; BGK tiles come first
   void* arena[2];
   const uint8_t* flags;
+  const uint32_t* bbm;     // per-node bounce-back masks, at the same offsets as the flags
   int32_t step;            // step index inside the chunk
 };
 
@@ -916,6 +1248,10 @@ __device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, c
   p.cav_gather = 1;
   p.nprog = 0;
   p.prog_planes = 0;
+  p.oflow = nullptr;
+  p.oflow_n = 0;
+  p.any_wall = 1;
+  p.bbm = bp.bbm + d.flag_off;
   p.mrt_custom = d.mrt_custom;
   for (int k = 0; k < 5; k++) {
     p.cav[0][k] = d.cav[0][k];
@@ -927,16 +1263,21 @@ __device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, c
   return p;
 }
 
-template <class S>
+// One launch per collision model and time step: the BGK launch walks the
+// BGK tiles, the MRT launch the MRT tiles, so each kernel keeps only its own
+// collision in registers.  Tiles are 256 consecutive nodes of one domain, so
+// the domain's parameters are workgroup-uniform.
+template <class S, int COLL>
 __global__ void __launch_bounds__(256) batch_kernel(const BatchParams bp) {
   using C = typename StoreOps<S>::C;
-  const uint32_t nt = *bp.ntiles;
-  for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+  const uint32_t t0 = COLL == COLL_BGK ? 0u : bp.ntiles[0];
+  const uint32_t t1 = COLL == COLL_BGK ? bp.ntiles[0] : bp.ntiles[1];
+  for (uint32_t t = t0 + blockIdx.x; t < t1; t += gridDim.x) {
     const BatchTile tl = bp.tiles[t];
     const BatchDomain& d = bp.doms[tl.dom];
     if (bp.step >= d.active_steps) continue;
     const int parity = (d.parity + bp.step) & 1;
-    const StepParams p = batch_step_params<S>(bp, d, parity);
+    StepParams p = batch_step_params<S>(bp, d, parity);
     const uint32_t n = tl.first + threadIdx.x;
     if (n >= p.N) continue;
     const uint32_t nxy = (uint32_t)p.nx * (uint32_t)p.ny;
@@ -944,13 +1285,15 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchParams bp) {
     const uint32_t rem = n - (uint32_t)z * nxy;
     const int y = (int)(rem / (uint32_t)p.nx);
     const int x = (int)(rem - (uint32_t)y * (uint32_t)p.nx);
-    const bool wall = p.flags[n] != 0;
+    const uint32_t m = bp.bbm[d.flag_off + n];
+    const bool wall = (m & 1u) != 0u;
+    const uint32_t wbits = d.wall_bb ? m : (m & 1u);
+    const uint32_t bo = n * (uint32_t)sizeof(S);
+    const NodeNbrs a = node_nbrs<S>(p, x, y, z);
     C f[Q];
-    gather_node<S, MODE_FUSED>(p, x, y, z, n, wall, f);
-    if (d.coll == COLL_MRT)
-      finish_node<S, COLL_MRT, MODE_FUSED>(p, x, y, z, n, wall, f);
-    else
-      finish_node<S, COLL_BGK, MODE_FUSED>(p, x, y, z, n, wall, f);
+    gather_bulk<S, MODE_FUSED>(p, bo, a, wbits, f);
+    collide_node<S, COLL, MODE_FUSED>(p, n, wall, f);
+    store_bulk<S, MODE_FUSED>(p, bo, a, wbits, f);
   }
 }
 

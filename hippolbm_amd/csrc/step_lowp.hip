@@ -4,11 +4,18 @@
 
 namespace hlbm {
 
-template <class S, int COLL, int MODE, int NPT>
+template <class S, int COLL, int MODE, int NPT, bool WIDE>
 static hipError_t go(dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
   hipError_t e = hipSuccess;
+  if constexpr (MODE == MODE_AA_EVEN || MODE == MODE_AA_ODD) {
+    if (p.oflow && !p.skip_program) {  // outflow values first: the step below updates nodes in place
+      hipLaunchKernelGGL((outflow_capture_kernel<S, MODE>), dim3((p.oflow_n + 255) / 256), dim3(256), 0, s, p);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+  }
   if (MODE != MODE_AA_FIX) {  // AA_FIX only touches program-plane nodes
-    hipLaunchKernelGGL((step_kernel<S, COLL, MODE, NPT>), grid, block, 0, s, p);
+    hipLaunchKernelGGL((step_kernel<S, COLL, MODE, NPT, WIDE>), grid, block, 0, s, p);
     e = hipGetLastError();
   }
   if (e != hipSuccess || MODE == MODE_COLLIDE || !p.prog_planes) return e;
@@ -18,7 +25,8 @@ static hipError_t go(dim3 grid, dim3 block, const StepParams& p, hipStream_t s) 
 
 template <class S, int COLL, int MODE>
 static hipError_t go_npt(int npt, dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
-  return npt == 2 ? go<S, COLL, MODE, 2>(grid, block, p, s) : go<S, COLL, MODE, 1>(grid, block, p, s);
+  if (wide_offsets(p, sizeof(S))) return go<S, COLL, MODE, 1, true>(grid, block, p, s);
+  return npt == 2 ? go<S, COLL, MODE, 2, false>(grid, block, p, s) : go<S, COLL, MODE, 1, false>(grid, block, p, s);
 }
 
 template <class S>
@@ -54,21 +62,26 @@ hipError_t launch_step_lowp(int storage, int coll, int mode, int npt, dim3 grid,
   }
 }
 
+template <class S>
+static hipError_t batch_pair(unsigned blocks, const BatchParams& bp, hipStream_t s) {
+  hipLaunchKernelGGL((batch_kernel<S, COLL_BGK>), dim3(blocks), dim3(256), 0, s, bp);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((batch_kernel<S, COLL_MRT>), dim3(blocks), dim3(256), 0, s, bp);
+  return hipGetLastError();
+}
+
 hipError_t launch_batch_lowp(int storage, unsigned blocks, const BatchParams& bp, hipStream_t s) {
   switch (storage) {
     case 1:
-      hipLaunchKernelGGL((batch_kernel<float>), dim3(blocks), dim3(256), 0, s, bp);
-      break;
+      return batch_pair<float>(blocks, bp, s);
     case 2:
-      hipLaunchKernelGGL((batch_kernel<_Float16>), dim3(blocks), dim3(256), 0, s, bp);
-      break;
+      return batch_pair<_Float16>(blocks, bp, s);
     case 3:
-      hipLaunchKernelGGL((batch_kernel<bf16s>), dim3(blocks), dim3(256), 0, s, bp);
-      break;
+      return batch_pair<bf16s>(blocks, bp, s);
     default:
       return hipErrorInvalidValue;
   }
-  return hipGetLastError();
 }
 
 }  // namespace hlbm

@@ -206,8 +206,52 @@ __global__ void __launch_bounds__(256) hbm_copy_kernel(uint4* __restrict__ dst, 
 }
 
 // ---------------------------------------------------------------------------
+// Bounce-back mask of the bulk kernel (wall_bounce_back, bounce_back.hpp:163-185):
+// bit 0 = the node is a wall; for a fluid node, bit i (1..18) = the upstream
+// node x - c_i (periodic wrap applied) is a wall.  An upstream node outside a
+// non-periodic face sets no bit: the face bounce-back covers it.
+// ---------------------------------------------------------------------------
+__global__ void k_build_bbm(const uint8_t* flags, int nx, int ny, int nz, int px, int py, int pz, uint32_t* bbm) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= nx || y >= ny) return;
+  const size_t idx = ((size_t)z * ny + y) * nx + x;
+  uint32_t m = 0;
+  if (flags[idx]) {
+    m = 1u;
+  } else {
+    static_for<1, Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      int sx = x - CX(i), sy = y - CY(i), sz = z - CZ(i);
+      if (sx < 0 || sx >= nx) {
+        if (!px) return;
+        sx = sx < 0 ? nx - 1 : 0;
+      }
+      if (sy < 0 || sy >= ny) {
+        if (!py) return;
+        sy = sy < 0 ? ny - 1 : 0;
+      }
+      if (sz < 0 || sz >= nz) {
+        if (!pz) return;
+        sz = sz < 0 ? nz - 1 : 0;
+      }
+      if (flags[((size_t)sz * ny + sy) * nx + sx]) m |= 1u << i;
+    });
+  }
+  bbm[idx] = m;
+}
+
+// ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
+hipError_t launch_build_bbm(const uint8_t* flags, int nx, int ny, int nz, const int* per, uint32_t* bbm,
+                            hipStream_t s) {
+  dim3 b(64, 4, 1), g((nx + 63) / 64, (ny + 3) / 4, nz);
+  hipLaunchKernelGGL(k_build_bbm, g, b, 0, s, flags, nx, ny, nz, per[0], per[1], per[2], bbm);
+  return hipGetLastError();
+}
+
 hipError_t launch_rasterize(uint8_t* flags, int nx, int ny, int nz, double dx, const ObstacleDev* obs, int nobs,
                             hipStream_t s) {
   dim3 b(64, 4, 1), g((nx + 63) / 64, (ny + 3) / 4, nz);

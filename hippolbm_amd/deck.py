@@ -42,7 +42,9 @@ HOT_OPERATORS = (
     "wall_bounce_back", "cavity_z_0", "cavity_z_l", "lid_driven_cavity", "neumann",
 )
 BC_OPERATORS = ("pre_bounce_back", "post_bounce_back", "wall_bounce_back", "cavity_z_0", "cavity_z_l",
-                "lid_driven_cavity", "neumann", "update_ghost", "nop")
+                "lid_driven_cavity", "neumann", "update_ghost", "nop",
+                # open boundaries with no counterpart in the reference (porous channel inlet / outlet)
+                "pressure_inlet", "outflow")
 PLANE_NAMES = ("plan_xy_0", "plan_xy_l", "plan_xz_0", "plan_xz_l", "plan_yz_0", "plan_yz_l")
 
 
@@ -145,6 +147,12 @@ class Deck:
                     for r in args["regions"]:
                         if r not in PLANE_NAMES:
                             raise DeckError(f"Traversal name not allowed: {r}")
+                if name in ("pressure_inlet", "outflow"):
+                    if "regions" not in args or (name == "pressure_inlet" and "rho" not in args):
+                        raise DeckError(f"{name}: regions{' and rho' if name == 'pressure_inlet' else ''} required")
+                    for r in args["regions"]:
+                        if r not in PLANE_NAMES:
+                            raise DeckError(f"Traversal name not allowed: {r}")
                 if name in ("cavity_z_0", "cavity_z_l") and "U" not in args:
                     raise DeckError(f"{name}: U is required")
         ids = sorted(self.obstacles)
@@ -217,6 +225,13 @@ def from_dict(doc: dict, name: str = "") -> Deck:
     if isinstance(coll, list):
         coll = _items(coll)[0][0]
     d = Deck(cell_dims=cell, bounds=bounds, periodic=per, lbm=lbm, collision=str(coll), name=name)
+    # extension (no reference slot: mrt.hpp hard-codes its rates): a top-level
+    # `mrt_rates: [s1, s2, s4, s9, s10, s13, s16, weps, wepsj, wxx]`
+    if "mrt_rates" in doc:
+        rates = [_num(v) for v in doc["mrt_rates"]]
+        if len(rates) != 10:
+            raise DeckError("mrt_rates needs 10 values: s1 s2 s4 s9 s10 s13 s16 weps wepsj wxx")
+        d.mrt_rates = rates
     for blk, attr in (("pre_stream_bcs", "pre_stream_bcs"), ("post_stream_bcs", "post_stream_bcs"),
                       ("boundary_conditions", "boundary_conditions")):
         lst = []
@@ -226,6 +241,8 @@ def from_dict(doc: dict, name: str = "") -> Deck:
                 args["U"] = _vec(args["U"])
             if "regions" in args:
                 args["regions"] = list(args["regions"])
+            if "rho" in args:
+                args["rho"] = _num(args["rho"])
             lst.append((k, args))
         setattr(d, attr, lst)
     for k, v in _items(doc.get("set_distributions")):

@@ -165,6 +165,19 @@ class Domain:
         check("hlbm_timestep", lib().hlbm_timestep(self.h, ctypes.byref(t)))
         return t.value
 
+    def autotune(self, reps=5, stream=None):
+        """Pick the fastest launch shape for this domain (cached per shape);
+        returns the chosen shape and the per-step times."""
+        t = _lib.TuneInfo()
+        s = ctypes.c_void_p(stream) if stream else None
+        check("hlbm_autotune", lib().hlbm_autotune(self.h, int(reps), s, ctypes.byref(t)))
+        return {"block": (t.block_x, t.block_y), "npt": t.npt, "xcd_remap": t.xcd_remap,
+                "candidates": t.candidates, "ms_best": t.ms_best, "ms_default": t.ms_default}
+
+    def set_launch(self, block=(128, 2), npt=1, xcd_remap=1):
+        check("hlbm_set_launch", lib().hlbm_set_launch(self.h, int(block[0]), int(block[1]), int(npt),
+                                                       int(xcd_remap)))
+
     def bytes(self):
         b = ctypes.c_uint64(0)
         check("hlbm_domain_bytes", lib().hlbm_domain_bytes(self.h, ctypes.byref(b)))
@@ -202,6 +215,8 @@ def bc_entries(deck):
             a = dict(args)
             if name in ("lid_driven_cavity", "neumann"):
                 a["U"] = tuple(u / c for u in args["U"])
+            if name == "pressure_inlet":  # the density travels in U[0]
+                a["U"] = (float(args["rho"]), 0.0, 0.0)
             out.append((stage, name, a))
     return out
 

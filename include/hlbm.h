@@ -56,7 +56,16 @@ enum hlbm_bc_kind {
   HLBM_BC_CAVITY_Z_0 = 4,        /* "cavity_z_0"        src/bcs/cavity.cu:66-99             */
   HLBM_BC_CAVITY_Z_L = 5,        /* "cavity_z_l"        src/bcs/cavity.cu:66-99             */
   HLBM_BC_LID_DRIVEN_CAVITY = 6, /* "lid_driven_cavity" src/bcs/lid_driven_cavity.cu:69-93  */
-  HLBM_BC_NEUMANN = 7            /* "neumann"           src/bcs/neuman.cu:69-122            */
+  HLBM_BC_NEUMANN = 7,           /* "neumann"           src/bcs/neuman.cu:69-122            */
+  /* Open-boundary operators with no counterpart in the reference (the porous
+   * channel's inlet and outlet); post_stream_bcs or boundary_conditions only,
+   * on the planes of a non-periodic axis.
+   *   pressure_inlet: Zou/He density boundary, U[0] = prescribed density,
+   *     zero tangential velocity (the neumann formulas with u_n from rho).
+   *   outflow: zero-gradient outlet, the populations entering from outside
+   *     copy their post-stream values at the first interior node. */
+  HLBM_BC_PRESSURE_INLET = 8,    /* "pressure_inlet"                                        */
+  HLBM_BC_OUTFLOW = 9            /* "outflow"                                               */
 };
 /* operator lists of the compute loop (data/config/config_compute_loop.msp:28-36) */
 enum hlbm_bc_stage { HLBM_STAGE_PRE_STREAM = 0, HLBM_STAGE_POST_STREAM = 1, HLBM_STAGE_BOUNDARY = 2 };
@@ -124,6 +133,11 @@ int hlbm_domain_bytes(const hlbm_domain* h, uint64_t* bytes);
 int hlbm_set_bcs(hlbm_domain* h, const hlbm_bc* list, int32_t n);
 /* Change tau / Fext (lbm_parameters) without recreating the domain. */
 int hlbm_set_params(hlbm_domain* h, double tau, const double fext[3]);
+/* Select the collision operator the `collision` slot names: HLBM_BGK
+ * (src/collision/bgk.cu:77-94) or HLBM_MRT (src/collision/mrt.cu:71-89);
+ * mrt_rates = NULL keeps the reference's hard-coded rates (mrt.hpp:44-47),
+ * otherwise [s1 s2 s4 s9 s10 s13 s16 weps wepsj wxx]. */
+int hlbm_set_collision(hlbm_domain* h, int32_t collision, const double* mrt_rates);
 
 /* ---- initial state ----------------------------------------------------- */
 /* init_obst: every node fluid (src/grid/init_obst.cu:54-60). */
@@ -240,6 +254,28 @@ int hlbm_batch_result(hlbm_batch* b, int64_t id, double* rho, double* u, double*
 int hlbm_batch_info_get(hlbm_batch* b, hlbm_batch_info* out);
 /* The library's steady clock, seconds (same clock as hlbm_batch_times). */
 double hlbm_clock(void);
+
+/* ---- launch-shape autotuner (no counterpart in the reference) -----------
+ * The step kernels take a tile shape (block_x nodes along x by block_y rows),
+ * a count of nodes per thread along y (npt, 1 or 2) and an XCD-aware tile
+ * order.  hlbm_autotune times every candidate on scratch memory (the domain's
+ * populations, phase and step count are left as they were: A/B domains write
+ * only the buffer the next step overwrites; AA domains work on a temporary
+ * copy), keeps the fastest for this domain and caches it per (dims, storage,
+ * collision, pattern, boundary program, device) so later domains of the same
+ * shape skip the search (out->candidates == 0).  Results never depend on the
+ * shape: every node runs the same arithmetic. */
+typedef struct {
+  int32_t block_x, block_y, npt, xcd_remap; /* chosen shape */
+  int32_t candidates;                       /* shapes timed; 0 = taken from the cache */
+  int32_t pad;
+  double ms_best;                           /* per step, chosen shape */
+  double ms_default;                        /* per step, the default 128x2 / npt 1 / remap */
+} hlbm_tune_info;
+
+int hlbm_autotune(hlbm_domain* h, int32_t reps, void* stream, hlbm_tune_info* out);
+/* Set the shape by hand: block_x*block_y a multiple of 64 and <= 256, npt 1|2. */
+int hlbm_set_launch(hlbm_domain* h, int32_t block_x, int32_t block_y, int32_t npt, int32_t xcd_remap);
 
 /* ---- measurement helper (no counterpart in the reference) --------------
  * Times `reps` streaming copies of `bytes` bytes between two scratch buffers

@@ -766,17 +766,29 @@ void orc_lid_driven_cavity(void* h, int plane, double Ux, double Uy, double Uz) 
 // neumann (Zou/He velocity), src/bcs/neuman.cu:69-122 and
 // src/bcs/include/hippoLBM/bcs/neumann.hpp:38-258. U in LBM units.
 // ---------------------------------------------------------------------------
-void orc_neumann(void* h, int plane, double ux, double uy, double uz) {
+// The density (pressure) form has no counterpart in the reference: rho_in is
+// prescribed, the tangential velocity is zero and the normal velocity follows
+// from the known-population sum K of the velocity form, rho (1 -+ u_n) = K.
+static double zou_he_density(double K, double& un, bool high, bool pressure, double rho_in) {
+  if (pressure) {
+    un = high ? K / rho_in - 1. : 1. - K / rho_in;
+    return rho_in;
+  }
+  return high ? K / (1. + un) : K / (1. - un);
+}
+
+static void zou_he(void* h, int plane, double ux, double uy, double uz, bool pressure, double rho_in) {
   Grid* g = (Grid*)h;
   const auto& pl = g->plane[plane];
+  if (pressure) ux = uy = uz = 0.0;
   for (size_t k = 0; k < pl.size(); k++) {
     const size_t idx = pl[k];
     if (g->obst[idx] != FLUID) continue;
 #define f(q) g->F(idx, q)
     if (plane == 5) {  // plan_yz_l -> neumann_x_l
-      const double rho = (f(3) + f(4) + f(5) + f(6) + f(15) + f(17) + f(18) + f(16) + f(0) +
-                          2. * (f(1) + f(7) + f(9) + f(11) + f(13))) /
-                         (1. + ux);
+      const double rho = zou_he_density(f(3) + f(4) + f(5) + f(6) + f(15) + f(17) + f(18) + f(16) + f(0) +
+                                            2. * (f(1) + f(7) + f(9) + f(11) + f(13)),
+                                        ux, true, pressure, rho_in);
       const double nyx = (1. / 2.) * (f(3) + f(15) + f(17) - (f(4) + f(18) + f(16))) - (1. / 3.) * rho * uy;
       const double nzx = (1. / 2.) * (f(5) + f(18) + f(15) - (f(6) + f(17) + f(16))) - (1. / 3.) * rho * uz;
       f(2) = f(1) - (1. / 3.) * rho * ux;
@@ -785,9 +797,9 @@ void orc_neumann(void* h, int plane, double ux, double uy, double uz) {
       f(12) = f(11) + (1. / 6.) * rho * (-ux - uz) + nzx;
       f(14) = f(13) + (1. / 6.) * rho * (-ux + uz) - nzx;
     } else if (plane == 4) {  // plan_yz_0 -> neumann_x_0
-      const double rho = (f(3) + f(4) + f(5) + f(6) + f(15) + f(17) + f(18) + f(16) + f(0) +
-                          2. * (f(2) + f(10) + f(8) + f(14) + f(12))) /
-                         (1. - ux);
+      const double rho = zou_he_density(f(3) + f(4) + f(5) + f(6) + f(15) + f(17) + f(18) + f(16) + f(0) +
+                                            2. * (f(2) + f(10) + f(8) + f(14) + f(12)),
+                                        ux, false, pressure, rho_in);
       const double nyx = (1. / 2.) * (f(3) + f(15) + f(17) - (f(4) + f(18) + f(16))) - (1. / 3.) * rho * uy;
       const double nzx = (1. / 2.) * (f(5) + f(18) + f(15) - (f(6) + f(17) + f(16))) - (1. / 3.) * rho * uz;
       f(1) = f(2) + (1. / 3.) * rho * ux;
@@ -796,9 +808,9 @@ void orc_neumann(void* h, int plane, double ux, double uy, double uz) {
       f(11) = f(12) + (1. / 6.) * rho * (ux + uz) - nzx;
       f(13) = f(14) + (1. / 6.) * rho * (ux - uz) + nzx;
     } else if (plane == 3) {  // plan_xz_l -> neumann_y_l
-      const double rho = (f(1) + f(2) + f(5) + f(6) + f(11) + f(13) + f(14) + f(12) + f(0) +
-                          2. * (f(3) + f(7) + f(10) + f(15) + f(17))) /
-                         (1. + uy);
+      const double rho = zou_he_density(f(1) + f(2) + f(5) + f(6) + f(11) + f(13) + f(14) + f(12) + f(0) +
+                                            2. * (f(3) + f(7) + f(10) + f(15) + f(17)),
+                                        uy, true, pressure, rho_in);
       const double nxy = (1. / 2.) * (f(1) + f(11) + f(13) - (f(2) + f(14) + f(12))) - (1. / 3.) * rho * ux;
       const double nzy = (1. / 2.) * (f(5) + f(11) + f(14) - (f(6) + f(13) + f(12))) - (1. / 3.) * rho * uz;
       f(4) = f(3) - (1. / 3.) * rho * uy;
@@ -807,9 +819,9 @@ void orc_neumann(void* h, int plane, double ux, double uy, double uz) {
       f(16) = f(15) + (1. / 6.) * rho * (-uy - uz) + nzy;
       f(18) = f(17) + (1. / 6.) * rho * (-uy + uz) - nzy;
     } else if (plane == 2) {  // plan_xz_0 -> neumann_y_0
-      const double rho = (f(1) + f(2) + f(5) + f(6) + f(11) + f(13) + f(14) + f(12) + f(0) +
-                          2. * (f(4) + f(9) + f(8) + f(18) + f(16))) /
-                         (1. - uy);
+      const double rho = zou_he_density(f(1) + f(2) + f(5) + f(6) + f(11) + f(13) + f(14) + f(12) + f(0) +
+                                            2. * (f(4) + f(9) + f(8) + f(18) + f(16)),
+                                        uy, false, pressure, rho_in);
       const double nxy = (1. / 2.) * (f(1) + f(11) + f(13) - (f(2) + f(14) + f(12))) - (1. / 3.) * rho * ux;
       const double nzy = (1. / 2.) * (f(5) + f(11) + f(14) - (f(6) + f(13) + f(12))) - (1. / 3.) * rho * uz;
       f(3) = f(4) + (1. / 3.) * rho * uy;
@@ -818,9 +830,9 @@ void orc_neumann(void* h, int plane, double ux, double uy, double uz) {
       f(15) = f(16) + (1. / 6.) * rho * (uy + uz) - nzy;
       f(17) = f(18) + (1. / 6.) * rho * (uy - uz) + nzy;
     } else if (plane == 1) {  // plan_xy_l -> neumann_z_l
-      const double rho = (f(0) + f(1) + f(2) + f(3) + f(4) + f(7) + f(9) + f(10) + f(8) +
-                          2. * (f(5) + f(11) + f(14) + f(15) + f(18))) /
-                         (1. + uz);
+      const double rho = zou_he_density(f(0) + f(1) + f(2) + f(3) + f(4) + f(7) + f(9) + f(10) + f(8) +
+                                            2. * (f(5) + f(11) + f(14) + f(15) + f(18)),
+                                        uz, true, pressure, rho_in);
       const double nxz = (1. / 2.) * (f(1) + f(7) + f(9) - (f(2) + f(10) + f(8))) - (1. / 3.) * rho * ux;
       const double nyz = (1. / 2.) * (f(3) + f(7) + f(10) - (f(4) + f(9) + f(8))) - (1. / 3.) * rho * uy;
       f(6) = f(5) - (1. / 3.) * rho * uz;
@@ -829,9 +841,9 @@ void orc_neumann(void* h, int plane, double ux, double uy, double uz) {
       f(17) = f(18) + (1. / 6.) * rho * (-uz + uy) - nyz;
       f(16) = f(15) + (1. / 6.) * rho * (-uz - uy) + nyz;
     } else {  // plane 0, plan_xy_0 -> neumann_z_0
-      const double rho = (f(0) + f(1) + f(2) + f(3) + f(4) + f(7) + f(9) + f(10) + f(8) +
-                          2. * (f(6) + f(13) + f(12) + f(17) + f(16))) /
-                         (1. - uz);
+      const double rho = zou_he_density(f(0) + f(1) + f(2) + f(3) + f(4) + f(7) + f(9) + f(10) + f(8) +
+                                            2. * (f(6) + f(13) + f(12) + f(17) + f(16)),
+                                        uz, false, pressure, rho_in);
       const double nxz = (1. / 2.) * (f(1) + f(7) + f(9) - (f(2) + f(10) + f(8))) - (1. / 3.) * rho * ux;
       const double nyz = (1. / 2.) * (f(3) + f(7) + f(10) - (f(4) + f(9) + f(8))) - (1. / 3.) * rho * uy;
       f(5) = f(6) + (1. / 3.) * rho * uz;
@@ -841,6 +853,30 @@ void orc_neumann(void* h, int plane, double ux, double uy, double uz) {
       f(18) = f(17) + (1. / 6.) * rho * (uz - uy) + nyz;
     }
 #undef f
+  }
+}
+
+void orc_neumann(void* h, int plane, double ux, double uy, double uz) { zou_he(h, plane, ux, uy, uz, false, 1.0); }
+
+void orc_pressure_inlet(void* h, int plane, double rho_in) { zou_he(h, plane, 0.0, 0.0, 0.0, true, rho_in); }
+
+// outflow (no counterpart in the reference): at the fluid nodes of a face, the
+// populations entering from outside copy the current (post-stream, post-BC)
+// values of the node one layer inside.
+void orc_outflow(void* h, int plane) {
+  Grid* g = (Grid*)h;
+  const auto& pl = g->plane[plane];
+  const int dim = plane < 2 ? 2 : (plane < 4 ? 1 : 0);
+  const int inward = (plane & 1) ? -1 : 1;
+  const long step = dim == 0 ? 1 : (dim == 1 ? (long)g->L[0] : (long)g->L[0] * g->L[1]);
+  for (size_t k = 0; k < pl.size(); k++) {
+    const size_t idx = pl[k];
+    if (g->obst[idx] != FLUID) continue;
+    const size_t nid = (size_t)((long)idx + inward * step);
+    for (int i = 1; i < Q; i++) {
+      const int cd = dim == 0 ? EX[i] : (dim == 1 ? EY[i] : EZ[i]);
+      if (cd == inward) g->F(idx, i) = g->F(nid, i);
+    }
   }
 }
 

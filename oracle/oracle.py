@@ -79,6 +79,8 @@ def lib():
             "orc_wall_bounce_back": (None, [P]),
             "orc_lid_driven_cavity": (None, [P, I, D, D, D]),
             "orc_neumann": (None, [P, I, D, D, D]),
+            "orc_pressure_inlet": (None, [P, I, D]),
+            "orc_outflow": (None, [P, I]),
             "orc_simulation_state": (None, [P, DP]),
         }
         for name, (res, args) in sig.items():
@@ -204,6 +206,12 @@ class Oracle:
     def neumann(self, plane, U):
         self.L.orc_neumann(self.h, PLANES[plane], U[0], U[1], U[2])
 
+    def pressure_inlet(self, plane, rho):
+        self.L.orc_pressure_inlet(self.h, PLANES[plane], float(rho))
+
+    def outflow(self, plane):
+        self.L.orc_outflow(self.h, PLANES[plane])
+
     def simulation_state(self):
         out = np.zeros(3)
         self.L.orc_simulation_state(self.h, _dp(out))
@@ -278,6 +286,12 @@ class OracleRun:
             U = [u / c for u in args["U"]]
             for r in args["regions"]:
                 o.neumann(r, U)
+        elif name == "pressure_inlet":
+            for r in args["regions"]:
+                o.pressure_inlet(r, args["rho"])
+        elif name == "outflow":
+            for r in args["regions"]:
+                o.outflow(r)
         elif name in ("nop", "update_ghost"):
             if name == "update_ghost":
                 o.update_ghost()

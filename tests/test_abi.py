@@ -47,6 +47,7 @@ def test_struct_layout_matches_header(tmp_path):
         'printf("%zu %zu %zu %zu\\n", sizeof(hlbm_solve_desc), offsetof(hlbm_solve_desc, tau),'
         ' offsetof(hlbm_solve_desc, cavity_u), offsetof(hlbm_solve_desc, obstacles));\n'
         'printf("%zu %zu %zu\\n", sizeof(hlbm_batch_desc), sizeof(hlbm_batch_info), offsetof(hlbm_batch_info, gpu_ms));\n'
+        'printf("%zu %zu\\n", sizeof(hlbm_tune_info), offsetof(hlbm_tune_info, ms_default));\n'
         "return 0;}\n")
     exe = tmp_path / "layout"
     subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
@@ -64,6 +65,8 @@ def test_struct_layout_matches_header(tmp_path):
     assert sd == [ctypes.sizeof(S), S.tau.offset, S.cavity_u.offset, S.obstacles.offset]
     bd = [int(v) for v in lines[4].split()]
     assert bd == [ctypes.sizeof(bt.BatchDesc), ctypes.sizeof(bt.BatchInfo), bt.BatchInfo.gpu_ms.offset]
+    tu = [int(v) for v in lines[5].split()]
+    assert tu == [ctypes.sizeof(_lib.TuneInfo), _lib.TuneInfo.ms_default.offset]
 
 
 def test_null_arguments_are_rejected_without_a_device():

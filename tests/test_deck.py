@@ -1,0 +1,78 @@
+"""Deck front end (no GPU): the open-boundary operators and the MRT-rate
+extension parse and validate, and the seeded workload generators are
+deterministic and well formed."""
+import numpy as np
+import pytest
+
+from hippolbm_amd import deck as dk
+from hippolbm_amd import workloads
+from hippolbm_amd.engine import bc_entries
+
+BASE = """
+do_domain:
+  - domain:
+     bounds: [[0,0,0],[9,7,5]]
+     cell_dims: [9,7,5]
+     periodic: [false, true, false]
+set_lbm_parameters:
+  - lbm_parameters:
+     tau: 0.7
+     nuth: 0.0666666666666667
+collision: mrt
+"""
+
+
+def test_open_boundary_operators_parse():
+    d = dk.loads_msp(BASE + """
+mrt_rates: [1.2, 1.2, 1.2, 1.4285714285714286, 1.2, 1.4285714285714286, 1.2, 0, -7.5396825396825395, 0]
+boundary_conditions:
+  - pressure_inlet:
+     rho: 1.001
+     regions: [plan_yz_0]
+  - outflow:
+     regions: [plan_yz_l]
+""")
+    assert d.boundary_conditions == [("pressure_inlet", {"rho": 1.001, "regions": ["plan_yz_0"]}),
+                                     ("outflow", {"regions": ["plan_yz_l"]})]
+    assert d.mrt_rates[3] == pytest.approx(1 / 0.7)
+    e = bc_entries(d)
+    assert e[0] == ("boundary_conditions", "pressure_inlet",
+                    {"rho": 1.001, "regions": ["plan_yz_0"], "U": (1.001, 0.0, 0.0)})
+
+
+@pytest.mark.parametrize("extra,msg", [
+    ("boundary_conditions:\n  - pressure_inlet:\n     regions: [plan_yz_0]\n", "rho"),
+    ("boundary_conditions:\n  - outflow:\n     regions: [plan_xx_0]\n", "Traversal"),
+    ("mrt_rates: [1, 2, 3]\n", "mrt_rates"),
+])
+def test_bad_open_boundary_decks_are_rejected(extra, msg):
+    with pytest.raises(dk.DeckError, match=msg):
+        dk.loads_msp(BASE + extra)
+
+
+def test_sphere_packing_is_seeded_and_non_overlapping():
+    a = workloads.pack_spheres((64, 32, 32), 20, (2.0, 5.0), seed=3, margin=(2.0, 0.0, 0.0))
+    b = workloads.pack_spheres((64, 32, 32), 20, (2.0, 5.0), seed=3, margin=(2.0, 0.0, 0.0))
+    assert all(np.array_equal(p[0], q[0]) and p[1] == q[1] for p, q in zip(a, b))
+    for i, (c, r) in enumerate(a):
+        assert 2.0 + r <= c[0] <= 63 - 2.0 - r and r <= c[1] <= 31 - r and r <= c[2] <= 31 - r
+        for c2, r2 in a[i + 1:]:
+            assert np.linalg.norm(c - c2) >= r + r2
+
+
+def test_porous_channel_deck_matches_the_configuration():
+    d = workloads.porous_channel()
+    assert d.nodes == (256, 128, 128) and len(d.obstacles) == 150
+    radii = [o["radius"] for o in d.obstacles.values()]
+    assert 5.0 <= min(radii) and max(radii) <= 12.0
+    assert d.collision == "mrt" and d.mrt_rates[3] == pytest.approx(1 / 0.7) and d.mrt_rates[0] == 1.2
+    assert d.pre_stream_bcs == [("pre_bounce_back", {}), ("wall_bounce_back", {})]
+    assert [n for n, _ in d.boundary_conditions] == ["pressure_inlet", "outflow"]
+    p = d.params
+    assert p["tau"] == pytest.approx(0.7) and p["celerity"] == pytest.approx(1.0)
+
+
+def test_poiseuille_deck_force_in_lattice_units():
+    d = workloads.poiseuille((32, 8, 16))
+    assert d.params["Fext"][0] == pytest.approx(1e-6)
+    assert d.periodic == (True, True, False)

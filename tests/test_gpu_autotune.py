@@ -1,0 +1,52 @@
+"""Launch-shape autotuner: it leaves the domain's state alone, every shape it
+can pick gives the same populations bit for bit, and a second domain of the
+same shape takes the cached choice."""
+import sys
+
+import numpy as np
+import pytest
+
+from hippolbm_amd.engine import Simulation
+
+from test_gpu_parity import _couette_deck, _porous_deck, seeded_equilibrium
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("storage,pattern", [("f64", "ab"), ("f16", "ab"), ("f32", "aa")])
+def test_autotune_keeps_state_and_results(storage, pattern):
+    d = _couette_deck(40)
+    f0 = seeded_equilibrium(d.nodes, 5)
+    ref = Simulation(d, storage=storage, init_f=f0, pattern=pattern)
+    ref.run(9)
+    sim = Simulation(d, storage=storage, init_f=f0, pattern=pattern)
+    sim.run(4)
+    info = sim.dom.autotune(reps=2)
+    say(storage, pattern, info)
+    assert info["block"][0] * info["block"][1] % 64 == 0 and info["npt"] in (1, 2)
+    sim.run(5)
+    assert np.array_equal(sim.dom.get_f(), ref.dom.get_f())
+    # the same shape again: taken from the cache
+    other = Simulation(d, storage=storage, init_f=f0, pattern=pattern)
+    again = other.dom.autotune(reps=2)
+    assert again["candidates"] == 0 and again["block"] == info["block"] and again["npt"] == info["npt"]
+    for s in (ref, sim, other):
+        s.dom.close()
+
+
+@pytest.mark.parametrize("shape", [((64, 1), 1, 0), ((64, 4), 2, 1), ((256, 1), 2, 0), ((128, 1), 1, 1)])
+def test_every_launch_shape_gives_the_same_populations(shape):
+    d = _porous_deck(37, 21, 19, 5, 7, "bgk")
+    a = Simulation(d, storage="f64")
+    b = Simulation(d, storage="f64")
+    b.dom.set_launch(*shape)
+    a.run(13)
+    b.run(13)
+    assert np.array_equal(a.dom.get_f(), b.dom.get_f())
+    a.dom.close()
+    b.dom.close()

@@ -1,0 +1,69 @@
+"""The C++ operator layer end to end, as the reference's regression tests run:
+`hlbm_run deck.msp` executes initialize / compute_loop / checker, the
+check_macro_quantities pin of every regression deck passes (exit status 0,
+"Test Pass."), and the final simulation_state equals the oracle's."""
+import glob
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as orc
+from hippolbm_amd import deck as dk
+
+from conftest import DECKS
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUN = os.path.join(ROOT, "hippolbm_amd", "hlbm_run")
+DECK_FILES = sorted(glob.glob(os.path.join(DECKS, "regression_*.msp")))
+
+
+def run(args):
+    print("hlbm_run", *args, flush=True)
+    sys.stdout.flush()
+    return subprocess.run([RUN] + args, capture_output=True, text=True, timeout=120)
+
+
+def final_state(out):
+    line = [l for l in out.splitlines() if l.startswith("final step")][-1].split()
+    return int(line[2]), float(line[4]), float(line[6]), float(line[8])
+
+
+@pytest.mark.parametrize("path", DECK_FILES, ids=[os.path.basename(p) for p in DECK_FILES])
+def test_regression_deck_passes_its_checker(path):
+    r = run([path, "--quiet"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Test Pass." in r.stdout
+    step, s, vmin, vmax = final_state(r.stdout)
+    d = dk.load_msp(path)
+    ref = orc.OracleRun(d)
+    ref.step(d.end_iteration)
+    o = ref.final_state()
+    assert step == d.end_iteration
+    assert abs(s - o[0]) <= 1e-12 * abs(o[0])
+    assert abs(vmin - o[1]) <= 1e-12 * max(abs(o[1]), 1e-300) + 1e-300
+    assert abs(vmax - o[2]) <= 1e-12 * abs(o[2])
+
+
+def test_log_table_and_aa_pattern():
+    path = os.path.join(DECKS, "regression_lbm_cavite_wall.msp")
+    r = run([path, "--pattern", "aa", "--steps", "7"])
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[0].split() == ["Step", "Time", "Mesh", "Size", "Sum(density)", "min(||V||)", "max(||V||)", "MLUPS"]
+    steps = [int(l.split()[0]) for l in lines[1:] if l.strip() and l.split()[0].isdigit()]
+    assert steps == list(range(0, 8))
+    assert "Test Pass." in r.stdout
+
+
+def test_failing_checker_exits_with_failure(tmp_path):
+    txt = open(os.path.join(DECKS, "regression_lbm_cavite.msp")).read().replace("density: 29791", "density: 29000")
+    deck = tmp_path / "bad_density.msp"
+    deck.write_text(txt)
+    r = run([str(deck), "--quiet"])
+    assert r.returncode != 0
+    assert "Regression test checker [macro quantities]" in r.stdout and "Sum of densities" in r.stdout

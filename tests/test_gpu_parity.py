@@ -95,16 +95,13 @@ post_stream_bcs:
 def seeded_equilibrium(nodes, seed, amp=1e-3):
     """rho = 1 + U(-amp, amp) (splitmix64 -> 53-bit uniform), u = 0, f = f_eq."""
     n = nodes[0] * nodes[1] * nodes[2]
-    state = np.uint64(seed)
-    out = np.empty(n)
     with np.errstate(over="ignore"):
-        for k in range(n):
-            state = np.uint64(state + np.uint64(0x9E3779B97F4A7C15))
-            z = state
-            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-            z = z ^ (z >> np.uint64(31))
-            out[k] = float(z >> np.uint64(11)) * (1.0 / 9007199254740992.0)
+        # state_k = seed + (k+1)*gamma (mod 2^64), then the splitmix64 finalizer
+        z = np.uint64(seed) + (np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    out = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
     rho = 1.0 + (2.0 * out - 1.0) * amp
     w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
     return (w[:, None] * rho[None, :]).reshape(19, nodes[2], nodes[1], nodes[0])
@@ -120,6 +117,29 @@ def test_seeded_couette_bitwise():
     sim.run(150)
     nbad, maxdiff, _ = compare(sim, ref)
     assert nbad == 0, (nbad, maxdiff)
+    sim.dom.close()
+
+
+def test_couette_64_cubed_2000_steps():
+    # the 64^3 Couette configuration: rho = 1 + U(-1e-3, 1e-3) seeded, u = 0,
+    # f = f_eq, 2000 steps; populations bit-identical to the oracle and the
+    # plane-averaged profile on the startup series (walls half a spacing
+    # outside the end nodes, moving at U (1 + 0.5/(L-1)))
+    from test_oracle import couette_startup
+    d = _couette_deck(64)
+    f0 = seeded_equilibrium(d.nodes, 20240601)
+    say("couette 64^3 seeded, 2000 steps")
+    ref = orc.OracleRun(d, init_f=f0)
+    sim = Simulation(d, storage="f64", init_f=f0)
+    ref.step(2000)
+    sim.run(2000)
+    nbad, maxdiff, _ = compare(sim, ref)
+    assert nbad == 0, (nbad, maxdiff)
+    rho, u = sim.dom.macro()
+    ux = u[0].mean(axis=(1, 2))
+    ana = couette_startup(np.arange(64) + 0.5, 2000, 64, 0.05 * (1 + 0.5 / 63), 0.1)
+    say("max |ux - startup series| / U", float(np.abs(ux - ana).max() / 0.05))
+    assert np.abs(ux - ana).max() < 1e-3 * 0.05
     sim.dom.close()
 
 
@@ -168,6 +188,75 @@ def test_porous_channel_bitwise(collision):
     nbad, maxdiff, _ = compare(sim, ref)
     assert nbad == 0, (nbad, maxdiff)
     sim.dom.close()
+
+
+def _channel_deck(nx, ny, nz, nballs, seed, collision="mrt", outlet="outflow"):
+    """Porous channel: density inlet at x=0, outflow (or a density outlet) at
+    x=L, bounce-back y/z walls, seeded non-overlapping balls, MRT with
+    s_nu = 1/0.7 and the other rates 1.2 (reference weps/wepsj/wxx)."""
+    rng = np.random.default_rng(seed)
+    balls = []
+    while len(balls) < nballs:
+        r = rng.uniform(1.5, 3.0)
+        c = np.array([rng.uniform(r + 3, nx - 4 - r), rng.uniform(0, ny - 1), rng.uniform(0, nz - 1)])
+        if all(np.linalg.norm(c - b[0]) > r + b[1] for b in balls):
+            balls.append((c, r))
+    out = ("  - outflow:\n     regions: [plan_yz_l]\n" if outlet == "outflow" else
+           "  - pressure_inlet:\n     rho: 1.0\n     regions: [plan_yz_l]\n")
+    lines = [f"""
+do_domain:
+  - domain:
+     bounds: [[0,0,0],[{nx - 1},{ny - 1},{nz - 1}]]
+     cell_dims: [{nx - 1},{ny - 1},{nz - 1}]
+     periodic: [false, false, false]
+set_lbm_parameters:
+  - lbm_parameters:
+     tau: 0.7
+     nuth: {0.2 / 3}
+collision: {collision}
+mrt_rates: [1.2, 1.2, 1.2, {1 / 0.7}, 1.2, {1 / 0.7}, 1.2, 0.0, {-475.0 / 63.0}, 0.0]
+pre_stream_bcs:
+  - pre_bounce_back
+  - wall_bounce_back
+post_stream_bcs:
+  - post_bounce_back
+boundary_conditions:
+  - pressure_inlet:
+     rho: 1.001
+     regions: [plan_yz_0]
+{out}set_obstacles:
+"""]
+    for k, (c, r) in enumerate(balls):
+        lines.append(f"  - register_solid_ball:\n     id: {k}\n     center: [{c[0]},{c[1]},{c[2]}]\n     radius: {r}\n")
+    return dk.loads_msp("".join(lines), "channel")
+
+
+@pytest.mark.parametrize("collision,outlet,pattern", [("mrt", "outflow", "ab"), ("bgk", "outflow", "ab"),
+                                                      ("mrt", "pressure", "ab"), ("mrt", "outflow", "aa")])
+def test_porous_channel_open_boundaries_bitwise(collision, outlet, pattern):
+    d = _channel_deck(40, 18, 16, 6, 21, collision, outlet)
+    say("channel", collision, outlet, pattern)
+    ref = orc.OracleRun(d)
+    sim = Simulation(d, storage="f64", pattern=pattern)
+    for n in (1, 12, 47):
+        ref.step(n)
+        sim.run(n)
+        nbad, maxdiff, _ = compare(sim, ref, fluid_only=True)
+        assert nbad == 0, (n, nbad, maxdiff)
+    if pattern == "ab":
+        nbad, maxdiff, _ = compare(sim, ref)
+        assert nbad == 0, (nbad, maxdiff)
+    rho, u = sim.dom.macro()
+    assert np.abs(rho[1:-1, 1:-1, 0][sim.dom.get_flags()[1:-1, 1:-1, 0] == -1] - 1.001).max() < 1e-12
+    assert u[0].max() > 0  # the inlet drives a flow
+    sim.dom.close()
+
+
+def test_outflow_after_an_operator_on_its_interior_layer_is_rejected():
+    d = _channel_deck(20, 10, 10, 0, 1, "bgk")
+    d.boundary_conditions.insert(0, ("lid_driven_cavity", {"U": (0.01, 0.0, 0.0), "regions": ["plan_xy_l"]}))
+    with pytest.raises(_lib.HLBMError):
+        Simulation(d, storage="f64")
 
 
 def test_operator_order_lid_before_post_bounce_back():

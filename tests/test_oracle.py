@@ -279,6 +279,98 @@ def test_zou_he_reproduces_wall_velocity(plane, axis, side):
     assert np.abs(j / r - U.reshape(3, 1, 1)).max() < 1e-13
 
 
+@pytest.mark.parametrize("plane,axis,side", [("plan_xy_0", 2, 0), ("plan_xy_l", 2, 1), ("plan_xz_0", 1, 0),
+                                             ("plan_xz_l", 1, 1), ("plan_yz_0", 0, 0), ("plan_yz_l", 0, 1)])
+def test_pressure_inlet_prescribes_density_and_normal_flow(plane, axis, side):
+    # the density form of Zou/He: afterwards the node's rho is the prescribed one,
+    # its tangential velocity is zero, and the normal momentum is what the
+    # known populations imply, rho_in*u_n = +-(rho_in - K)
+    rng = np.random.default_rng(16)
+    n = (6, 5, 7)
+    rho = 1 + rng.uniform(-0.01, 0.01, n[::-1])
+    u = rng.uniform(-0.02, 0.02, (3,) + n[::-1])
+    f = feq(rho, u) + rng.uniform(-1e-4, 1e-4, (19,) + n[::-1])
+    o = _oracle_with(f, (False, False, False))
+    rho_in = 1.0013
+    o.pressure_inlet(plane, rho_in)
+    g = o.node_f()
+    sl = [slice(None)] * 3
+    sl[2 - axis] = -1 if side else 0
+    sl = (slice(None),) + tuple(sl)
+    r, j = moments(g[sl])
+    assert np.abs(r - rho_in).max() < 1e-14
+    for d in range(3):
+        if d != axis:
+            assert np.abs(j[d]).max() < 1e-15
+    c = C[:, axis]
+    out = -1 if side == 0 else 1  # the direction leaving through the face
+    K = f[sl][c == 0].sum(0) + 2 * f[sl][c == out].sum(0)
+    un = (1 - K / rho_in) if side == 0 else (K / rho_in - 1)
+    assert np.abs(j[axis] - rho_in * un).max() < 1e-14
+    # nothing but the 5 incoming populations of the face nodes changed
+    inc = np.where(c == -out)[0]
+    keep = np.ones_like(g, dtype=bool)
+    for q in inc:
+        keep[(q,) + sl[1:]] = False
+    assert np.array_equal(g[keep], f[keep])
+
+
+@pytest.mark.parametrize("plane,axis,side", [("plan_xy_0", 2, 0), ("plan_xz_l", 1, 1), ("plan_yz_0", 0, 0),
+                                             ("plan_yz_l", 0, 1)])
+def test_outflow_copies_the_first_interior_layer(plane, axis, side):
+    rng = np.random.default_rng(17)
+    n = (6, 5, 7)
+    f = rng.uniform(0.0, 0.1, (19,) + n[::-1])
+    o = _oracle_with(f, (False, False, False))
+    o.outflow(plane)
+    g = o.node_f()
+    ref = f.copy()
+    face = [slice(None)] * 3
+    inner = [slice(None)] * 3
+    face[2 - axis] = -1 if side else 0
+    inner[2 - axis] = -2 if side else 1
+    inward = -1 if side else 1
+    for q in range(19):
+        if C[q, axis] == inward:
+            ref[(q,) + tuple(face)] = f[(q,) + tuple(inner)]
+    assert np.array_equal(g, ref)
+
+
+def test_pressure_driven_channel_profile():
+    # density 1.001 at x=0 and 1.0 at x=L (the density boundary on both faces),
+    # no-slip (Zou/He U=0) z plates, periodic y: the developed profile
+    # mid-channel is the plane Poiseuille parabola.  (With the zero-gradient
+    # outflow at x=L instead, the channel fills to the inlet density and the
+    # flow dies out: no steady closed form, so outflow is pinned above.)
+    nx, nz, tau = 24, 11, 0.9
+    o = orc.Oracle((nx, 3, nz), (False, True, False), 1.0)
+    o.init_obst()
+    o.set_distribution(1.0)
+    for _ in range(4000):
+        o.macro_variables()
+        o.bgk(tau)
+        o.streaming()
+        o.neumann("plan_xy_0", (0.0, 0.0, 0.0))
+        o.neumann("plan_xy_l", (0.0, 0.0, 0.0))
+        o.pressure_inlet("plan_yz_0", 1.001)
+        o.pressure_inlet("plan_yz_l", 1.0)
+    o.macro_variables()
+    ux = o.real(o.m1())[0, :, 1, nx // 2]
+    z = np.arange(nz)
+    H = nz - 1
+    shape = z * (H - z) / (H / 2) ** 2
+    assert ux.max() > 1e-4
+    assert np.abs(ux / ux.max() - shape).max() < 0.02
+    rho = o.real(o.m0())[:, 1, :]
+    assert np.abs(rho[1:-1, 0] - 1.001).max() < 1e-12  # the faces hold their densities
+    assert np.abs(rho[1:-1, -1] - 1.0).max() < 1e-12
+    # magnitude: u_max = G H^2 / (8 nu rho) with the interior pressure gradient G = d(rho/3)/dx
+    nu = (tau - 0.5) / 3
+    mid = nz // 2
+    G = (rho[mid, 1] - rho[mid, -2]) / 3 / (nx - 3)
+    assert abs(ux.max() - G * H * H / (8 * nu * rho[mid].mean())) < 0.02 * ux.max()
+
+
 def test_lid_driven_cavity_sets_equilibrium_at_local_density():
     rng = np.random.default_rng(8)
     n = (5, 5, 5)
@@ -337,6 +429,42 @@ def test_poiseuille_steady_profile():
     H = nz - 1
     ana = F / (2 * nu) * z * (H - z)
     assert np.abs(ux - ana).max() / ana.max() < 0.02
+
+
+def couette_startup(z, t, H, Uw, nu, terms=400):
+    """Startup plane Couette between a fixed wall at z=0 and a wall moving at Uw
+    at z=H (Fourier series of the finite channel)."""
+    s = Uw * z / H
+    for n in range(1, terms):
+        s = s - 2 * Uw / (n * np.pi) * (-1) ** (n + 1) * np.sin(n * np.pi * z / H) * np.exp(
+            -nu * (n * np.pi / H) ** 2 * t)
+    return s
+
+
+def test_couette_startup_transient_with_moving_wall_bounce_back():
+    # the 64^3 Couette configuration reduced to its z column: on-node bounce-back at
+    # z=0, cavity_z_l moving wall at z=63.  The bounce-back walls sit half a spacing
+    # outside the end nodes (H = nz), and cavity_z_l's factor (1 + 0.5/(L-1))
+    # (cavity.hpp:58-63) is the wall speed there: the profile after 2000 steps
+    # (nu t / H^2 = 0.05, far from steady) is the startup series
+    nz, tau, U, T = 64, 0.8, 0.05, 2000
+    o = orc.Oracle((3, 3, nz), (True, True, False), 1.0)
+    o.init_obst()
+    o.set_distribution(1.0)
+    for _ in range(T):
+        o.macro_variables()
+        o.bgk(tau)
+        o.pre_bounce_back()
+        o.cavity(1, (U, 0.0, 0.0))
+        o.streaming()
+        o.post_bounce_back()
+    o.macro_variables()
+    ux = o.real(o.m1())[0, :, 1, 1]
+    ana = couette_startup(np.arange(nz) + 0.5, T, nz, U * (1 + 0.5 / (nz - 1)), (tau - 0.5) / 3)
+    assert np.abs(ux - ana).max() < 2e-4 * U
+    # a wall at the end nodes themselves (H = nz - 1) misses by >1 % of U
+    wrong = couette_startup(np.arange(nz) * 1.0, T, nz - 1, U, (tau - 0.5) / 3)
+    assert np.abs(ux - wrong).max() > 1e-2 * U
 
 
 def test_couette_linear_profile():
