@@ -51,6 +51,8 @@ class DomainDesc(ctypes.Structure):
         ("fext", ctypes.c_double * 3),
         ("dx", ctypes.c_double),
         ("mrt_rates", ctypes.c_double * 10),
+        ("round_nearest", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
     ]
 
 

@@ -22,7 +22,7 @@ P = ctypes.c_void_p
 
 class BatchDesc(ctypes.Structure):
     _fields_ = [("storage", ctypes.c_int32), ("chunk_steps", ctypes.c_int32), ("use_graph", ctypes.c_int32),
-                ("pad", ctypes.c_int32), ("budget_bytes", ctypes.c_uint64)]
+                ("round_nearest", ctypes.c_int32), ("budget_bytes", ctypes.c_uint64)]
 
 
 class SolveDesc(ctypes.Structure):
@@ -154,10 +154,11 @@ class Solve:
 
 
 class Batch:
-    def __init__(self, storage="f16", budget_bytes=8 << 30, chunk_steps=200, use_graph=True):
+    def __init__(self, storage="f16", budget_bytes=8 << 30, chunk_steps=200, use_graph=True, rounding="stochastic"):
         _bind()
         d = BatchDesc()
         d.storage = _lib.STORAGE[storage]
+        d.round_nearest = 1 if rounding == "nearest" else 0
         d.chunk_steps = int(chunk_steps)
         d.use_graph = 1 if use_graph else 0
         d.budget_bytes = int(budget_bytes)

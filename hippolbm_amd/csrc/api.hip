@@ -103,6 +103,9 @@ static StepParams base_params(hlbm_domain* h) {
   if (!h->any_wall) p.wall_bb = 0;
   p.any_wall = h->any_wall ? 1 : 0;
   p.bbm = h->bbm;
+  // 16-bit storage: stochastic rounding keyed on (node, time step)
+  p.sr = (h->d.storage == HLBM_F16 || h->d.storage == HLBM_BF16) && !h->d.round_nearest ? 1 : 0;
+  p.seed = mix32((uint32_t)h->t * 0x9E3779B9u + 1u);
   return p;
 }
 

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -121,6 +122,8 @@ struct hlbm_batch {
   hipGraphExec_t graph = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool tiles_dirty = true;
+  int xcd_split = 1;
+  int sr = 0;  // 16-bit storage rounds stochastically
   int64_t chunks = 0, n_done = 0;
   double node_updates = 0, gpu_ms = 0;
 };
@@ -138,6 +141,8 @@ static StepParams single_params(hlbm_batch* b, const Solve& s, int parity) {
   p.flags = b->flags + s.flag_off;
   p.any_wall = 1;
   p.bbm = b->bbm + s.flag_off;  // built at admission
+  p.sr = b->sr;
+  p.seed = mix32((uint32_t)s.steps_done * 0x9E3779B9u + 1u);
   p.wall_bb = s.dom.wall_bb;
   p.plane_bb = 1;
   p.cav_on[0] = s.dom.cav_on[0];
@@ -176,6 +181,8 @@ static hipError_t chunk_launch(hlbm_batch* b, int step) {
   bp.flags = b->flags;
   bp.bbm = b->bbm;
   bp.step = step;
+  bp.xcd_split = b->xcd_split;
+  bp.sr = b->sr;
   if (b->d.storage == HLBM_F64) return launch_batch_f64(b->blocks, bp, b->stream);
   return launch_batch_lowp(b->d.storage, b->blocks, bp, b->stream);
 }
@@ -212,6 +219,8 @@ int hlbm_batch_create(const hlbm_batch_desc* desc, hlbm_batch** out) {
   hlbm_batch* b = new hlbm_batch();
   b->d = *desc;
   b->esize = desc->storage == HLBM_F64 ? 8 : (desc->storage == HLBM_F32 ? 4 : 2);
+  if (const char* xs = getenv("HLBM_BATCH_XCD_SPLIT")) b->xcd_split = atoi(xs) ? 1 : 0;
+  b->sr = (desc->storage == HLBM_F16 || desc->storage == HLBM_BF16) && !desc->round_nearest ? 1 : 0;
   // budget = 2 arenas of 19 planes + flags + masks: per node 38*esize + 1 + 4 bytes
   const uint64_t per_node = 38 * b->esize + 5;
   b->max_nodes = desc->budget_bytes / per_node;
@@ -289,7 +298,7 @@ int hlbm_batch_submit(hlbm_batch* b, const hlbm_solve_desc* s, int64_t* id) {
   const uint64_t N = (uint64_t)s->nodes[0] * s->nodes[1] * s->nodes[2];
   const uint64_t stride = round_up(N, 128);
   if (stride * 19 > b->arena_elems || N > b->max_nodes) return bfail(HLBM_ERR_NOMEM, "solve can never fit the budget");
-  if ((N - 1) * b->esize > 0xFFFFFFFFull)
+  if ((stride + N) * b->esize > 0xFFFFFFFFull)
     return bfail(HLBM_ERR_UNSUPPORTED, "solve too large for the batch kernel's 32-bit offsets; use a domain");
   Solve v;
   v.id = b->next_id++;
@@ -309,6 +318,8 @@ int hlbm_batch_submit(hlbm_batch* b, const hlbm_solve_desc* s, int64_t* id) {
   d.coll = s->collision == HLBM_MRT ? COLL_MRT : COLL_BGK;
   d.wall_bb = s->wall_bb && s->n_obstacles > 0 ? 1 : 0;
   d.mrt_custom = s->mrt_custom;
+  d.inv_nx = 1.0 / (double)d.nx;
+  d.inv_nxy = 1.0 / ((double)d.nx * (double)d.ny);
   d.tau = s->tau;
   for (int i = 0; i < Q; i++) d.ef[i] = CX(i) * s->fext[0] + CY(i) * s->fext[1] + CZ(i) * s->fext[2];
   for (int k = 0; k < 10; k++) d.mrt[k] = s->mrt_rates[k];
@@ -457,6 +468,7 @@ int hlbm_batch_run(hlbm_batch* b, int32_t* n_finished) {
     Solve& s = b->solves[b->running[k]];
     s.dom.active_steps = std::min(K, s.fused_left);
     s.dom.parity = s.parity;
+    s.dom.t0 = (uint32_t)s.steps_done;
     b->h_doms[k] = s.dom;
   }
   if (b->tiles_dirty) {

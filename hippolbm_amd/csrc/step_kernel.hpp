@@ -88,6 +88,8 @@ struct StepParams {
   int32_t any_wall;     // some node is a wall
   int32_t oflow_n;      // nodes per plane slot of oflow
   void* oflow;          // outflow values captured before an in-place AA step (C type), or null
+  uint32_t seed;        // stochastic rounding: hash of the time step (0 with sr = 0)
+  int32_t sr;           // 16-bit storage rounds stochastically
   double cav[2][5];
   double tau;
   double F[3];
@@ -106,31 +108,50 @@ struct bf16s {
 template <class S>
 struct StoreOps;
 
+// ld/st convert between the stored and the compute type with round-to-nearest-
+// even; st_sr rounds stochastically with the uniform 32-bit word r: the bits
+// the narrower format drops are replaced by random ones before truncation, so
+// a value is rounded up with probability equal to its distance from the lower
+// neighbour (unbiased, and exact for representable values).  FP16 stores
+// g * 2^13: the shifted populations are small (|g| < 0.1), and the scale keeps
+// them above FP16's subnormal range, where its 11-bit precision holds.
 template <>
 struct StoreOps<double> {
   using C = double;
   static constexpr bool shifted = false;
+  static constexpr bool narrow16 = false;
   static __device__ __forceinline__ C ld(const double* p, size_t i) { return p[i]; }
   static __device__ __forceinline__ void st(double* p, size_t i, C v) { p[i] = v; }
+  static __device__ __forceinline__ void st_sr(double* p, size_t i, C v, uint32_t) { p[i] = v; }
 };
 template <>
 struct StoreOps<float> {
   using C = float;
   static constexpr bool shifted = true;
+  static constexpr bool narrow16 = false;
   static __device__ __forceinline__ C ld(const float* p, size_t i) { return p[i]; }
   static __device__ __forceinline__ void st(float* p, size_t i, C v) { p[i] = v; }
+  static __device__ __forceinline__ void st_sr(float* p, size_t i, C v, uint32_t) { p[i] = v; }
 };
 template <>
 struct StoreOps<_Float16> {
   using C = float;
   static constexpr bool shifted = true;
-  static __device__ __forceinline__ C ld(const _Float16* p, size_t i) { return (float)p[i]; }
-  static __device__ __forceinline__ void st(_Float16* p, size_t i, C v) { p[i] = (_Float16)v; }
+  static constexpr bool narrow16 = true;
+  static constexpr float SCALE = 8192.0f, INV_SCALE = 1.0f / 8192.0f;  // 2^13, 2^-13
+  static __device__ __forceinline__ C ld(const _Float16* p, size_t i) { return (float)p[i] * INV_SCALE; }
+  static __device__ __forceinline__ void st(_Float16* p, size_t i, C v) { p[i] = (_Float16)(v * SCALE); }
+  static __device__ __forceinline__ void st_sr(_Float16* p, size_t i, C v, uint32_t r) {
+    // FP16 keeps 10 of FP32's 23 mantissa bits: dither the 13 dropped ones
+    const uint32_t u = (__float_as_uint(v * SCALE) + (r >> 19)) & 0xFFFFE000u;
+    p[i] = (_Float16)__uint_as_float(u);
+  }
 };
 template <>
 struct StoreOps<bf16s> {
   using C = float;
   static constexpr bool shifted = true;
+  static constexpr bool narrow16 = true;
   static __device__ __forceinline__ C ld(const bf16s* p, size_t i) {
     return __uint_as_float(((uint32_t)p[i].v) << 16);
   }
@@ -141,6 +162,41 @@ struct StoreOps<bf16s> {
     r.v = (uint16_t)(u >> 16);
     p[i] = r;
   }
+  static __device__ __forceinline__ void st_sr(bf16s* p, size_t i, C v, uint32_t r) {
+    bf16s o;
+    o.v = (uint16_t)((__float_as_uint(v) + (r >> 16)) >> 16);  // dither the 16 dropped bits
+    p[i] = o;
+  }
+};
+
+// Random words for stochastic rounding: one integer hash per (node, step), then
+// a Weyl sequence over the directions (the top bits, which the stores use, are
+// uniform for each direction).
+__host__ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
+  x ^= x >> 16;
+  x *= 0x7feb352du;
+  x ^= x >> 15;
+  x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+// The rounding word of direction i at a node: random with stochastic rounding,
+// else half an ulp of the dropped bits (round to nearest, ties away from zero).
+template <class S>
+struct Rounder {
+  uint32_t base = 0, inc = 0;
+  template <class P>
+  __device__ __forceinline__ Rounder(const P& p, uint32_t idx) {
+    if constexpr (StoreOps<S>::narrow16) {
+      if (p.sr) {
+        base = mix32(idx ^ p.seed);
+        inc = 0x9E3779B9u;
+      } else {
+        base = 0x80000000u;
+      }
+    }
+  }
+  __device__ __forceinline__ uint32_t word(int i) const { return base + (uint32_t)i * inc; }
 };
 
 // ---------------------------------------------------------------------------
@@ -788,9 +844,10 @@ __device__ __forceinline__ void store_node(const StepParams& p, int x, int y, in
   using C = typename O::C;
   S* __restrict__ fout = (S*)p.fout;
   const size_t st = p.stride;
+  const Rounder<S> R(p, idx);
   if constexpr (MODE == MODE_AA_EVEN) {
 #pragma unroll
-    for (int i = 0; i < Q; i++) O::st(fout, (size_t)OPP(i) * st + idx, f[i]);
+    for (int i = 0; i < Q; i++) O::st_sr(fout, (size_t)OPP(i) * st + idx, f[i], R.word(i));
   } else if constexpr (MODE == MODE_AA_ODD) {
     // push f*_i(x) to the natural slot of x + c_i; a non-periodic face or (with
     // wall_bounce_back) a wall downstream bounces it into slot opp(i) of x,
@@ -814,18 +871,18 @@ __device__ __forceinline__ void store_node(const StepParams& p, int x, int y, in
           constexpr int side = CZ(i) == -1 ? 0 : 1;  // face of the incoming direction opp(i)
           if (tz < 0 && p.cav_gather && p.cav_on[side]) v += C(p.cav[side][cavity_slot(OPP(i))]);
         }
-        O::st(fout, (size_t)OPP(i) * st + idx, v);
+        O::st_sr(fout, (size_t)OPP(i) * st + idx, v, R.word(i));
       } else {
         const uint32_t tidx = ((uint32_t)tz * (uint32_t)ny + (uint32_t)ty) * (uint32_t)nx + (uint32_t)tx;
         if (i != 0 && wbb && p.flags[tidx] != 0)
-          O::st(fout, (size_t)OPP(i) * st + idx, f[i]);
+          O::st_sr(fout, (size_t)OPP(i) * st + idx, f[i], R.word(i));
         else
-          O::st(fout, (size_t)i * st + tidx, f[i]);
+          O::st_sr(fout, (size_t)i * st + tidx, f[i], R.word(i));
       }
     });
   } else {
 #pragma unroll
-    for (int i = 0; i < Q; i++) O::st(fout, (size_t)i * st + idx, f[i]);
+    for (int i = 0; i < Q; i++) O::st_sr(fout, (size_t)i * st + idx, f[i], R.word(i));
   }
 }
 
@@ -975,12 +1032,13 @@ __device__ __forceinline__ void store_bulk(const StepParams& p, uint32_t bo, con
   S* __restrict__ fout = (S*)p.fout;
   const size_t st = p.stride;
   S* b = fout;
+  const Rounder<S> R(p, bo / (uint32_t)sizeof(S));
   bo = lane_offset(bo);
   if constexpr (MODE == MODE_AA_EVEN) {
     static_for([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       b = scalar_ptr(j ? b + st : b);
-      O::st(at_bytes(b, bo), 0, f[OPP(j)]);
+      O::st_sr(at_bytes(b, bo), 0, f[OPP(j)], R.word(OPP(j)));
     });
   } else if constexpr (MODE == MODE_AA_ODD) {
     // push to x + c_i; a face or (wall_bb) wall downstream bounces into slot opp(i) of x
@@ -989,7 +1047,7 @@ __device__ __forceinline__ void store_bulk(const StepParams& p, uint32_t bo, con
       static_for([&](auto ic) {
         constexpr int i = decltype(ic)::value;
         b = scalar_ptr(i ? b + st : b);
-        O::st(at_bytes(b, bo + nbr_offset<i, false>(a)), 0, f[i]);
+        O::st_sr(at_bytes(b, bo + nbr_offset<i, false>(a)), 0, f[i], R.word(i));
       });
     } else {
       static_for([&](auto ic) {
@@ -1002,9 +1060,9 @@ __device__ __forceinline__ void store_bulk(const StepParams& p, uint32_t bo, con
             constexpr int side = CZ(i) == -1 ? 0 : 1;  // face of the incoming direction opp(i)
             if (zf && p.cav_gather && p.cav_on[side]) v += C(p.cav[side][cavity_slot(OPP(i))]);
           }
-          O::st(at_bytes(fout + (size_t)OPP(i) * st, bo), 0, v);
+          O::st_sr(at_bytes(fout + (size_t)OPP(i) * st, bo), 0, v, R.word(i));
         } else {
-          O::st(at_bytes(fout + (size_t)i * st, bo + nbr_offset<i, false>(a)), 0, f[i]);
+          O::st_sr(at_bytes(fout + (size_t)i * st, bo + nbr_offset<i, false>(a)), 0, f[i], R.word(i));
         }
       });
     }
@@ -1012,7 +1070,7 @@ __device__ __forceinline__ void store_bulk(const StepParams& p, uint32_t bo, con
     static_for([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       b = scalar_ptr(j ? b + st : b);
-      O::st(at_bytes(b, bo), 0, f[j]);
+      O::st_sr(at_bytes(b, bo), 0, f[j], R.word(j));
     });
   }
 }
@@ -1199,10 +1257,12 @@ struct BatchDomain {
   int32_t active_steps;  // steps this domain takes in the current chunk
   int32_t parity;        // arena holding f* at the start of the chunk
   int32_t mrt_custom;
-  int32_t pad;
+  uint32_t t0;           // the domain's time steps done when the chunk starts (rounding seed)
   uint64_t f_off;        // element offset of population plane 0 in an arena
   uint64_t stride;       // elements between population planes
   uint64_t flag_off;     // byte offset in the flag arena
+  double inv_nx;         // 1/nx and 1/(nx*ny): node coordinates without integer division
+  double inv_nxy;
   double tau;
   double ef[Q];
   double cav[2][5];
@@ -1222,6 +1282,8 @@ struct BatchParams {
   const uint8_t* flags;
   const uint32_t* bbm;     // per-node bounce-back masks, at the same offsets as the flags
   int32_t step;            // step index inside the chunk
+  int32_t xcd_split;       // each XCD walks its own contiguous run of tiles
+  int32_t sr;              // 16-bit storage rounds stochastically
 };
 
 template <class S>
@@ -1250,6 +1312,8 @@ __device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, c
   p.prog_planes = 0;
   p.oflow = nullptr;
   p.oflow_n = 0;
+  p.sr = bp.sr;
+  p.seed = mix32((d.t0 + (uint32_t)bp.step) * 0x9E3779B9u + 1u);
   p.any_wall = 1;
   p.bbm = bp.bbm + d.flag_off;
   p.mrt_custom = d.mrt_custom;
@@ -1263,6 +1327,38 @@ __device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, c
   return p;
 }
 
+// Gather for a batch node (same result as gather_node), without branches: the
+// opposite directions (2k-1, 2k) share one scalar base (their first plane), so
+// a lane that bounces back (face or wall upstream) loads f*_opp(i)(x) in the
+// same instruction as its neighbours' pull, by selecting the other offset.
+// Needs (stride + N) * sizeof(S) < 2^32 (checked when a solve is submitted).
+template <class S>
+__device__ __forceinline__ void gather_select(const StepParams& p, uint32_t bo, const NodeNbrs& a, uint32_t wbits,
+                                              typename StoreOps<S>::C* f) {
+  using O = StoreOps<S>;
+  using C = typename O::C;
+  const S* __restrict__ fin = (const S*)p.fin;
+  const uint32_t stb = (uint32_t)(p.stride * sizeof(S));
+  bo = lane_offset(bo);
+  f[0] = O::ld(at_bytes(fin, bo), 0);
+  const S* b = fin;
+  static_for<1, Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    constexpr int lo = (i & 1) ? i : i - 1;  // first plane of the pair {i, opp(i)}
+    if constexpr (i & 1) b = scalar_ptr(b + (size_t)(i == 1 ? 1 : 2) * p.stride);
+    bool zf;
+    const bool bb = nbr_outside<i, true>(a, zf) || ((wbits >> i) & 1u);
+    const uint32_t pull = (i - lo) * stb + bo + nbr_offset<i, true>(a);
+    const uint32_t back = (OPP(i) - lo) * stb + bo;
+    C v = O::ld(at_bytes(b, bb ? back : pull), 0);
+    if constexpr (CZ(i) != 0) {
+      constexpr int side = CZ(i) == 1 ? 0 : 1;
+      if (p.cav_on[side]) v += (zf ? C(p.cav[side][cavity_slot(i)]) : C(0));
+    }
+    f[i] = v;
+  });
+}
+
 // One launch per collision model and time step: the BGK launch walks the
 // BGK tiles, the MRT launch the MRT tiles, so each kernel keeps only its own
 // collision in registers.  Tiles are 256 consecutive nodes of one domain, so
@@ -1270,9 +1366,20 @@ __device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, c
 template <class S, int COLL>
 __global__ void __launch_bounds__(256) batch_kernel(const BatchParams bp) {
   using C = typename StoreOps<S>::C;
-  const uint32_t t0 = COLL == COLL_BGK ? 0u : bp.ntiles[0];
-  const uint32_t t1 = COLL == COLL_BGK ? bp.ntiles[0] : bp.ntiles[1];
-  for (uint32_t t = t0 + blockIdx.x; t < t1; t += gridDim.x) {
+  uint32_t t0 = COLL == COLL_BGK ? 0u : bp.ntiles[0];
+  uint32_t t1 = COLL == COLL_BGK ? bp.ntiles[0] : bp.ntiles[1];
+  uint32_t first = blockIdx.x, stride = gridDim.x;
+  if (bp.xcd_split && (gridDim.x & 7u) == 0u) {
+    // workgroups go to the 8 XCDs round-robin: XCD k (= blockIdx % 8) takes the
+    // k-th eighth of the tiles, so neighbouring tiles share that XCD's L2
+    const uint32_t k = blockIdx.x & 7u, n = t1 - t0, per = (n + 7u) / 8u;
+    const uint32_t a = t0 + min(n, k * per), e = t0 + min(n, (k + 1u) * per);
+    t0 = a;
+    t1 = e;
+    first = blockIdx.x >> 3;
+    stride = gridDim.x >> 3;
+  }
+  for (uint32_t t = t0 + first; t < t1; t += stride) {
     const BatchTile tl = bp.tiles[t];
     const BatchDomain& d = bp.doms[tl.dom];
     if (bp.step >= d.active_steps) continue;
@@ -1280,18 +1387,20 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchParams bp) {
     StepParams p = batch_step_params<S>(bp, d, parity);
     const uint32_t n = tl.first + threadIdx.x;
     if (n >= p.N) continue;
+    // floor((n + 1/2) / d) from the rounded reciprocal is exact: the quotient's
+    // distance to an integer, >= 1/(2d), exceeds the relative rounding error 2^-52
     const uint32_t nxy = (uint32_t)p.nx * (uint32_t)p.ny;
-    const int z = (int)(n / nxy);
-    const uint32_t rem = n - (uint32_t)z * nxy;
-    const int y = (int)(rem / (uint32_t)p.nx);
-    const int x = (int)(rem - (uint32_t)y * (uint32_t)p.nx);
+    const uint32_t zu = (uint32_t)(((double)n + 0.5) * d.inv_nxy);
+    const uint32_t rem = n - zu * nxy;
+    const uint32_t yu = (uint32_t)(((double)rem + 0.5) * d.inv_nx);
+    const int z = (int)zu, y = (int)yu, x = (int)(rem - yu * (uint32_t)p.nx);
     const uint32_t m = bp.bbm[d.flag_off + n];
     const bool wall = (m & 1u) != 0u;
     const uint32_t wbits = d.wall_bb ? m : (m & 1u);
     const uint32_t bo = n * (uint32_t)sizeof(S);
     const NodeNbrs a = node_nbrs<S>(p, x, y, z);
     C f[Q];
-    gather_bulk<S, MODE_FUSED>(p, bo, a, wbits, f);
+    gather_select<S>(p, bo, a, wbits, f);
     collide_node<S, COLL, MODE_FUSED>(p, n, wall, f);
     store_bulk<S, MODE_FUSED>(p, bo, a, wbits, f);
   }

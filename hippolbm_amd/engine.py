@@ -32,8 +32,11 @@ class Domain:
     """One single-GPU subdomain (replaces domain + define_grid_3dq19 + fields)."""
 
     def __init__(self, nodes, periodic, tau, fext=(0.0, 0.0, 0.0), storage="f64", collision="bgk", dx=1.0,
-                 mrt_rates=None, pattern="ab"):
+                 mrt_rates=None, pattern="ab", rounding="stochastic"):
         d = DomainDesc()
+        if rounding not in ("stochastic", "nearest"):
+            raise ValueError("rounding is 'stochastic' or 'nearest'")
+        d.round_nearest = 1 if rounding == "nearest" else 0
         for k in range(3):
             d.nodes[k] = int(nodes[k])
             d.periodic[k] = 1 if periodic[k] else 0
@@ -224,12 +227,13 @@ def bc_entries(deck):
 class Simulation:
     """A deck run on the GPU engine with the reference's operator order."""
 
-    def __init__(self, deck, storage="f64", init_f=None, pattern="ab"):
+    def __init__(self, deck, storage="f64", init_f=None, pattern="ab", rounding="stochastic"):
         p = deck.params
         self.deck = deck
         self.params = p
         self.dom = Domain(deck.nodes, deck.periodic, p["tau"], p["Fext"], storage=storage,
-                          collision=deck.collision, dx=deck.dx, mrt_rates=deck.mrt_rates, pattern=pattern)
+                          collision=deck.collision, dx=deck.dx, mrt_rates=deck.mrt_rates, pattern=pattern,
+                          rounding=rounding)
         self.dom.set_bcs(bc_entries(deck))
         # initialize (config_initialize.msp:21-27)
         self.dom.init_obst()

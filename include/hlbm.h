@@ -92,6 +92,10 @@ typedef struct {
   double fext[3];      /* LBMParameters::Fext_ (LBM units)                                     */
   double dx;           /* grid spacing, used by obstacle rasterization                         */
   double mrt_rates[10];/* s1, s2, s4, s9, s10, s13, s16, weps, wepsj, wxx                     */
+  int32_t round_nearest;/* FP16/BF16 storage: 0 = stochastic rounding of every stored
+                           population (the default: unbiased, so rounding errors do not
+                           accumulate in steady flow), 1 = round to nearest even          */
+  int32_t pad;
 } hlbm_domain_desc;
 
 typedef struct {
@@ -208,7 +212,7 @@ typedef struct {
   int32_t storage;      /* enum hlbm_storage, one per batch                         */
   int32_t chunk_steps;  /* steps per hlbm_batch_run call (graph length)             */
   int32_t use_graph;    /* 1: replay a captured HIP graph per chunk                 */
-  int32_t pad;
+  int32_t round_nearest;/* as hlbm_domain_desc.round_nearest                        */
   uint64_t budget_bytes;/* HBM for populations + flags of all resident solves       */
 } hlbm_batch_desc;
 

@@ -64,8 +64,9 @@ def test_batch_equals_single_domain(storage):
             assert np.array_equal(f, f1)
             assert np.array_equal(rho, r1)
             assert np.array_equal(u, u1)
-        else:  # FP32 arithmetic may contract differently in the two kernels
-            assert np.abs(f - f1).max() < 1e-5
+        else:  # FP32 arithmetic may contract differently in the two kernels, which can move a
+            # stored value by one FP16 step (1.5e-5 at |g| ~ 0.03)
+            assert np.abs(f - f1).max() < 5e-5
     b.close()
 
 
@@ -112,7 +113,7 @@ def test_admission_under_a_small_budget():
     res = run_all(b, ids)
     for sid, s in zip(ids, solves):
         _, _, f1 = s.run_single("f16")
-        assert np.abs(res[sid][2] - f1).max() < 1e-5
+        assert np.abs(res[sid][2] - f1).max() < 5e-5
     b.close()
 
 

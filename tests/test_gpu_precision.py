@@ -68,3 +68,36 @@ def test_mass_conservation_f16():
     m1 = s.state()["sum_density"]
     assert abs(m1 - m0) / m0 < 1e-4
     s.dom.close()
+
+
+def test_stochastic_rounding_keeps_long_runs_accurate():
+    # 64^3 Couette, 2000 steps: with round-to-nearest the same rounding repeats
+    # every step in the quasi-steady flow and the FP16 error grows to ~1e-2;
+    # stochastic rounding keeps it at the 1e-3 target (a CPU column model of
+    # this flow gives 1.3e-2 and 4e-4)
+    d = deck(64)
+    ref = Simulation(d, storage="f64")
+    ref.run(2000)
+    _, u0 = ref.dom.macro()
+    ref.dom.close()
+    err = {}
+    for rounding in ("stochastic", "nearest"):
+        low = Simulation(d, storage="f16", rounding=rounding)
+        low.run(2000)
+        _, u1 = low.dom.macro()
+        low.dom.close()
+        err[rounding] = np.linalg.norm(u1 - u0) / np.linalg.norm(u0)
+        print(f"f16 {rounding}: rel L2(u) = {err[rounding]:.3e}", flush=True)
+    assert err["stochastic"] < 1e-3
+    assert err["nearest"] > 3 * err["stochastic"]
+
+
+def test_stochastic_rounding_is_deterministic():
+    d = deck(24)
+    a = Simulation(d, storage="f16")
+    b = Simulation(d, storage="f16")
+    a.run(37)
+    b.run(37)
+    assert np.array_equal(a.dom.get_f(), b.dom.get_f())
+    a.dom.close()
+    b.dom.close()
