@@ -268,6 +268,7 @@ def serve(batch, requests, keep_results=False, poll_sleep=0.0005):
     memory frees; returns per-request time-to-result (s) and aggregate figures."""
     import time
     t0 = clock()
+    gpu0 = batch.info()["gpu_ms"] * 1e-3
     pending = list(requests)
     inflight = {}
     lat = []
@@ -295,6 +296,7 @@ def serve(batch, requests, keep_results=False, poll_sleep=0.0005):
                     results[sid] = (spec, r)
     makespan = clock() - t0
     lat = np.array(lat)
+    gpu_s = batch.info()["gpu_ms"] * 1e-3 - gpu0
     return {"n": len(lat), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "mean_s": float(lat.mean()), "makespan_s": makespan, "node_updates": node_updates,
-            "mlups": node_updates / makespan / 1e6, "results": results}
+            "mlups": node_updates / makespan / 1e6, "gpu_busy": gpu_s / makespan, "results": results}

@@ -94,6 +94,7 @@ static StepParams base_params(hlbm_domain* h) {
   for (int d = 0; d < 3; d++) p.F[d] = h->d.fext[d];
   // c_i . F in the reference's order: ex*Fx + ey*Fy + ez*Fz (bgk.hpp:57, mrt.hpp:188)
   for (int i = 0; i < Q; i++) p.ef[i] = CX(i) * h->d.fext[0] + CY(i) * h->d.fext[1] + CZ(i) * h->d.fext[2];
+  set_fp32_constants(p);
   p.mrt_custom = h->d.mrt_custom;
   p.gx = (int)((h->nx + h->block.x - 1) / h->block.x);
   p.gy = (int)tiles_y(h);

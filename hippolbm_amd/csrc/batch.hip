@@ -6,7 +6,13 @@
 // and one MRT launch per step, captured once as a HIP graph), each walking its
 // part of the tile table of all running solves.
 // Admission runs the collide-only kernel on the new slice; retirement runs the
-// stream-only kernel, macro_variables, and copies the result out.
+// stream-only kernel and macro_variables, leaving rho/u in the solve's idle
+// arena half until hlbm_batch_result copies them out.
+//
+// hlbm_batch_run enqueues and returns: up to two chunks are in flight on the
+// main stream, so the host's admission / retirement work for the next chunk
+// overlaps the device's current one.  Pinned staging buffers (domain table,
+// tile table, obstacle lists) come in two sets, one per chunk in flight.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -88,6 +94,9 @@ struct Solve {
   uint64_t stride = 0, f_off = 0, f_len = 0, flag_off = 0, flag_len = 0;
   BatchDomain dom{};
   double t_submit = 0, t_admit = 0, t_done = 0;
+  bool retiring = false;      // retirement enqueued; DONE once `ev` has completed
+  hipEvent_t ev = nullptr;
+  int res_arena = 0;          // arena whose slice holds rho (N) and u (3N), as doubles
   std::vector<double> rho, u, f;
 };
 
@@ -110,17 +119,26 @@ struct hlbm_batch {
   BatchDomain* d_doms = nullptr;
   BatchTile* d_tiles = nullptr;
   uint32_t* d_ntiles = nullptr;
-  BatchDomain* h_doms = nullptr;  // pinned staging
-  BatchTile* h_tiles = nullptr;
-  uint32_t* h_ntiles = nullptr;
+  // pinned staging, one set per chunk in flight
+  BatchDomain* h_doms_set[2] = {nullptr, nullptr};
+  BatchTile* h_tiles_set[2] = {nullptr, nullptr};
+  uint32_t* h_ntiles_set[2] = {nullptr, nullptr};
+  ObstacleDev* h_obs[2] = {nullptr, nullptr};
+  ObstacleDev* d_obs[2] = {nullptr, nullptr};
+  size_t obs_cap = 0, obs_used[2] = {0, 0};
   uint64_t max_doms = 0, max_tiles = 0;
-  double* m_tmp = nullptr;  // rho + u of one retiring solve (4 * max_nodes doubles)
-  double* f_tmp = nullptr;  // one FP64 population plane
+  double* f_tmp = nullptr;  // one FP64 population plane (keep_populations)
   uint64_t max_nodes = 0;
   unsigned blocks = 2048;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;       // chunks, admission, retirement
+  hipStream_t copy_stream = nullptr;  // result read-back
   hipGraphExec_t graph = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  struct ChunkEvents {
+    hipEvent_t start = nullptr, end = nullptr;
+    bool pending = false;
+  } cev[2];
+  int64_t run_index = 0;  // chunks enqueued
+  std::vector<int64_t> retiring;
   bool tiles_dirty = true;
   int xcd_split = 1;
   int sr = 0;  // 16-bit storage rounds stochastically
@@ -155,6 +173,7 @@ static StepParams single_params(hlbm_batch* b, const Solve& s, int parity) {
   }
   p.tau = s.dom.tau;
   for (int i = 0; i < Q; i++) p.ef[i] = s.dom.ef[i];
+  set_fp32_constants(p);
   for (int k = 0; k < 10; k++) p.mrt[k] = s.dom.mrt[k];
   p.gx = (s.dom.nx + 127) / 128;
   p.gy = (s.dom.ny + 1) / 2;
@@ -248,13 +267,18 @@ int hlbm_batch_create(const hlbm_batch_desc* desc, hlbm_batch** out) {
   if (hipMalloc((void**)&b->d_doms, b->max_doms * sizeof(BatchDomain)) != hipSuccess) return cleanup("domains");
   if (hipMalloc((void**)&b->d_tiles, b->max_tiles * sizeof(BatchTile)) != hipSuccess) return cleanup("tiles");
   if (hipMalloc((void**)&b->d_ntiles, 2 * sizeof(uint32_t)) != hipSuccess) return cleanup("tile count");
-  if (hipHostMalloc((void**)&b->h_doms, b->max_doms * sizeof(BatchDomain)) != hipSuccess) return cleanup("pinned");
-  if (hipHostMalloc((void**)&b->h_tiles, b->max_tiles * sizeof(BatchTile)) != hipSuccess) return cleanup("pinned");
-  if (hipHostMalloc((void**)&b->h_ntiles, 2 * sizeof(uint32_t)) != hipSuccess) return cleanup("pinned");
-  if (hipStreamCreate(&b->stream) != hipSuccess) return cleanup("stream");
-  if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess) return cleanup("events");
-  b->h_ntiles[0] = b->h_ntiles[1] = 0;
-  if (hipMemcpy(b->d_ntiles, b->h_ntiles, 2 * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+  for (int k = 0; k < 2; k++) {
+    if (hipHostMalloc((void**)&b->h_doms_set[k], b->max_doms * sizeof(BatchDomain)) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_tiles_set[k], b->max_tiles * sizeof(BatchTile)) != hipSuccess ||
+        hipHostMalloc((void**)&b->h_ntiles_set[k], 2 * sizeof(uint32_t)) != hipSuccess)
+      return cleanup("pinned");
+    if (hipEventCreate(&b->cev[k].start) != hipSuccess || hipEventCreate(&b->cev[k].end) != hipSuccess)
+      return cleanup("events");
+    b->h_ntiles_set[k][0] = b->h_ntiles_set[k][1] = 0;
+  }
+  if (hipStreamCreate(&b->stream) != hipSuccess || hipStreamCreate(&b->copy_stream) != hipSuccess)
+    return cleanup("stream");
+  if (hipMemcpy(b->d_ntiles, b->h_ntiles_set[0], 2 * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
     return cleanup("tile count init");
   b->fl_pop.init(b->arena_elems);
   b->fl_flag.init(b->flag_bytes);
@@ -273,15 +297,21 @@ int hlbm_batch_create(const hlbm_batch_desc* desc, hlbm_batch** out) {
 int hlbm_batch_destroy(hlbm_batch* b) {
   if (!b) return HLBM_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->copy_stream) (void)hipStreamSynchronize(b->copy_stream);
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   for (void* p : {b->arena[0], b->arena[1], (void*)b->flags, (void*)b->bbm, (void*)b->d_doms, (void*)b->d_tiles,
-                  (void*)b->d_ntiles, (void*)b->m_tmp, (void*)b->f_tmp})
+                  (void*)b->d_ntiles, (void*)b->f_tmp, (void*)b->d_obs[0], (void*)b->d_obs[1]})
     if (p) (void)hipFree(p);
-  for (void* p : {(void*)b->h_doms, (void*)b->h_tiles, (void*)b->h_ntiles})
-    if (p) (void)hipHostFree(p);
-  if (b->ev0) (void)hipEventDestroy(b->ev0);
-  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  for (int k = 0; k < 2; k++) {
+    for (void* p : {(void*)b->h_doms_set[k], (void*)b->h_tiles_set[k], (void*)b->h_ntiles_set[k], (void*)b->h_obs[k]})
+      if (p) (void)hipHostFree(p);
+    if (b->cev[k].start) (void)hipEventDestroy(b->cev[k].start);
+    if (b->cev[k].end) (void)hipEventDestroy(b->cev[k].end);
+  }
+  for (auto& kv : b->solves)
+    if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->copy_stream) (void)hipStreamDestroy(b->copy_stream);
   delete b;
   return HLBM_OK;
 }
@@ -322,6 +352,12 @@ int hlbm_batch_submit(hlbm_batch* b, const hlbm_solve_desc* s, int64_t* id) {
   d.inv_nxy = 1.0 / ((double)d.nx * (double)d.ny);
   d.tau = s->tau;
   for (int i = 0; i < Q; i++) d.ef[i] = CX(i) * s->fext[0] + CY(i) * s->fext[1] + CZ(i) * s->fext[2];
+  d.has_force = 0;
+  for (int i = 0; i < Q; i++) {
+    d.ef3f[i] = 3.0f * (float)d.ef[i];
+    if (d.ef[i] != 0.0) d.has_force = 1;
+  }
+  d.omega_f = (float)(1.0 / d.tau);
   for (int k = 0; k < 10; k++) d.mrt[k] = s->mrt_rates[k];
   const double L = d.nz;  // cavity.hpp:58-63
   for (int side = 0; side < 2; side++) {
@@ -365,12 +401,31 @@ static int admit_one(hlbm_batch* b, Solve& s) {
       if (inside) od.push_back(v);
     }
     if (!od.empty()) {
-      ObstacleDev* dob = nullptr;
-      if (hipMalloc(&dob, od.size() * sizeof(ObstacleDev)) != hipSuccess) return bfail(HLBM_ERR_NOMEM, "obstacles");
-      hipError_t e = hipMemcpyAsync(dob, od.data(), od.size() * sizeof(ObstacleDev), hipMemcpyHostToDevice, st);
-      if (e == hipSuccess) e = launch_rasterize(b->flags + s.flag_off, s.dom.nx, s.dom.ny, s.dom.nz, dx, dob, (int)od.size(), st);
-      if (e == hipSuccess) e = hipStreamSynchronize(st);
-      (void)hipFree(dob);
+      // the obstacle list goes through this chunk's pinned staging set
+      const int set = (int)(b->run_index & 1);
+      if (b->obs_used[set] + od.size() > b->obs_cap) {
+        // grow both sets (rare): wait until no queued copy reads the old ones
+        if (hipStreamSynchronize(st) != hipSuccess) return bfail(HLBM_ERR_HIP, "sync");
+        const size_t cap = std::max<size_t>(2 * b->obs_cap, b->obs_used[set] + od.size() + 1024);
+        for (int k = 0; k < 2; k++) {
+          ObstacleDev *h = nullptr, *dv = nullptr;
+          if (hipHostMalloc((void**)&h, cap * sizeof(ObstacleDev)) != hipSuccess ||
+              hipMalloc((void**)&dv, cap * sizeof(ObstacleDev)) != hipSuccess)
+            return bfail(HLBM_ERR_NOMEM, "obstacle staging");
+          if (b->h_obs[k]) (void)hipHostFree(b->h_obs[k]);
+          if (b->d_obs[k]) (void)hipFree(b->d_obs[k]);
+          b->h_obs[k] = h;
+          b->d_obs[k] = dv;
+        }
+        b->obs_cap = cap;
+        b->obs_used[0] = b->obs_used[1] = 0;
+      }
+      ObstacleDev* hs = b->h_obs[set] + b->obs_used[set];
+      ObstacleDev* ds = b->d_obs[set] + b->obs_used[set];
+      std::memcpy(hs, od.data(), od.size() * sizeof(ObstacleDev));
+      b->obs_used[set] += od.size();
+      hipError_t e = hipMemcpyAsync(ds, hs, od.size() * sizeof(ObstacleDev), hipMemcpyHostToDevice, st);
+      if (e == hipSuccess) e = launch_rasterize(b->flags + s.flag_off, s.dom.nx, s.dom.ny, s.dom.nz, dx, ds, (int)od.size(), st);
       if (e != hipSuccess) return bfail(HLBM_ERR_HIP, "rasterize");
     }
   }
@@ -396,6 +451,10 @@ static int admit_one(hlbm_batch* b, Solve& s) {
   return HLBM_OK;
 }
 
+// Enqueue a finished solve's read-out: stream-only step (the reference's
+// post-stream state), then macro_variables into the idle arena half of its
+// slice (rho at the slice start, u after it).  The solve is DONE when the
+// recorded event completes; hlbm_batch_result copies the fields out.
 static int retire_one(hlbm_batch* b, Solve& s) {
   hipStream_t st = b->stream;
   int par = s.parity;
@@ -403,18 +462,14 @@ static int retire_one(hlbm_batch* b, Solve& s) {
     if (single_launch(b, s, MODE_STREAM, par) != hipSuccess) return bfail(HLBM_ERR_HIP, "stream");
     par ^= 1;
   }
-  if (!b->m_tmp && hipMalloc((void**)&b->m_tmp, b->max_nodes * 4 * 8) != hipSuccess) return bfail(HLBM_ERR_NOMEM, "tmp");
   const void* fcur = (const char*)b->arena[par] + s.f_off * b->esize;
+  s.res_arena = par ^ 1;
+  double* rho = (double*)((char*)b->arena[s.res_arena] + s.f_off * b->esize);  // 4N doubles <= 19N * esize bytes
   // walls keep rho = 0 (macro_variables never writes them)
-  if (hipMemsetAsync(b->m_tmp, 0, (size_t)s.N * 8, st) != hipSuccess) return bfail(HLBM_ERR_HIP, "memset");
-  if (launch_macro(b->d.storage, fcur, s.stride, s.N, b->flags + s.flag_off, b->m_tmp, b->m_tmp + s.N, st) != hipSuccess)
+  if (hipMemsetAsync(rho, 0, (size_t)s.N * 8, st) != hipSuccess) return bfail(HLBM_ERR_HIP, "memset");
+  if (launch_macro(b->d.storage, fcur, s.stride, s.N, b->flags + s.flag_off, rho, rho + s.N, st) != hipSuccess)
     return bfail(HLBM_ERR_HIP, "macro");
-  s.rho.resize(s.N);
-  s.u.resize(3 * (size_t)s.N);
-  if (hipMemcpyAsync(s.rho.data(), b->m_tmp, s.N * 8, hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipMemcpyAsync(s.u.data(), b->m_tmp + s.N, 3 * (size_t)s.N * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
-    return bfail(HLBM_ERR_HIP, "copy out");
-  if (s.desc.keep_populations) {
+  if (s.desc.keep_populations) {  // test and I/O path: read everything back now
     if (!b->f_tmp && hipMalloc((void**)&b->f_tmp, b->max_nodes * 8) != hipSuccess) return bfail(HLBM_ERR_NOMEM, "tmp");
     s.f.resize((size_t)19 * s.N);
     for (int i = 0; i < Q; i++) {
@@ -423,19 +478,61 @@ static int retire_one(hlbm_batch* b, Solve& s) {
           hipMemcpyAsync(s.f.data() + (size_t)i * s.N, b->f_tmp, s.N * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
         return bfail(HLBM_ERR_HIP, "populations out");
     }
+    if (hipStreamSynchronize(st) != hipSuccess) return bfail(HLBM_ERR_HIP, "sync");
   }
-  if (hipStreamSynchronize(st) != hipSuccess) return bfail(HLBM_ERR_HIP, "sync");
-  b->fl_pop.release(s.f_off, s.f_len);
-  b->fl_flag.release(s.flag_off, s.flag_len);
-  s.state = HLBM_SOLVE_DONE;
-  s.t_done = now_s();
-  b->n_done++;
+  if (!s.ev && hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) != hipSuccess) return bfail(HLBM_ERR_HIP, "event");
+  if (hipEventRecord(s.ev, st) != hipSuccess) return bfail(HLBM_ERR_HIP, "event");
+  s.retiring = true;
+  b->retiring.push_back(s.id);
+  return HLBM_OK;
+}
+
+// Retirements whose device work has completed become DONE (wait = block on the oldest).
+static int poll_retiring(hlbm_batch* b, bool wait, int* finished) {
+  std::vector<int64_t> keep;
+  for (size_t k = 0; k < b->retiring.size(); k++) {
+    Solve& s = b->solves[b->retiring[k]];
+    hipError_t q = (wait && k == 0) ? hipEventSynchronize(s.ev) : hipEventQuery(s.ev);
+    if (q == hipSuccess) {
+      s.retiring = false;
+      s.state = HLBM_SOLVE_DONE;
+      s.t_done = now_s();
+      b->n_done++;
+      if (finished) (*finished)++;
+    } else if (q == hipErrorNotReady) {
+      keep.push_back(b->retiring[k]);
+    } else {
+      return bfail(HLBM_ERR_HIP, std::string("retirement: ") + hipGetErrorString(q));
+    }
+  }
+  b->retiring.swap(keep);
+  return HLBM_OK;
+}
+
+// Account the device time of a completed chunk.
+static int finish_chunk(hlbm_batch* b, int set) {
+  auto& c = b->cev[set];
+  if (!c.pending) return HLBM_OK;
+  if (hipEventSynchronize(c.end) != hipSuccess) return bfail(HLBM_ERR_HIP, "chunk sync");
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, c.start, c.end);
+  b->gpu_ms += ms;
+  b->chunks++;
+  c.pending = false;
   return HLBM_OK;
 }
 
 int hlbm_batch_run(hlbm_batch* b, int32_t* n_finished) {
   if (!b) return bfail(HLBM_ERR_ARG, "NULL batch");
   int finished = 0;
+  // this chunk uses staging set `set`; the chunk that last used it (two calls
+  // ago) must be complete before the set is rewritten
+  const int set = (int)(b->run_index & 1);
+  int rc = finish_chunk(b, set);
+  if (rc) return rc;
+  b->obs_used[set] = 0;
+  rc = poll_retiring(b, false, &finished);
+  if (rc) return rc;
   // admission (first fit, in arrival order, backfilling smaller solves)
   for (auto it = b->queue.begin(); it != b->queue.end();) {
     Solve& s = b->solves[*it];
@@ -455,7 +552,7 @@ int hlbm_batch_run(hlbm_batch* b, int32_t* n_finished) {
     s.dom.f_off = foff;
     s.dom.stride = s.stride;
     s.dom.flag_off = goff;
-    int rc = admit_one(b, s);
+    rc = admit_one(b, s);
     if (rc) return rc;
     b->running.push_back(s.id);
     it = b->queue.erase(it);
@@ -463,53 +560,56 @@ int hlbm_batch_run(hlbm_batch* b, int32_t* n_finished) {
   }
   // per-domain chunk lengths and the tile table (BGK domains' tiles first)
   const int K = b->d.chunk_steps;
+  BatchDomain* h_doms = b->h_doms_set[set];
+  BatchTile* h_tiles = b->h_tiles_set[set];
+  uint32_t* h_ntiles = b->h_ntiles_set[set];
   uint32_t nt = 0, nbgk = 0;
   for (size_t k = 0; k < b->running.size(); k++) {
     Solve& s = b->solves[b->running[k]];
     s.dom.active_steps = std::min(K, s.fused_left);
     s.dom.parity = s.parity;
     s.dom.t0 = (uint32_t)s.steps_done;
-    b->h_doms[k] = s.dom;
+    h_doms[k] = s.dom;
   }
   if (b->tiles_dirty) {
     for (int pass = 0; pass < 2; pass++) {
       for (size_t k = 0; k < b->running.size(); k++) {
         const Solve& s = b->solves[b->running[k]];
         if (s.dom.coll != (pass == 0 ? COLL_BGK : COLL_MRT)) continue;
-        for (uint32_t first = 0; first < s.N; first += 256) b->h_tiles[nt++] = BatchTile{(uint32_t)k, first};
+        for (uint32_t first = 0; first < s.N; first += 256) h_tiles[nt++] = BatchTile{(uint32_t)k, first};
       }
       if (pass == 0) nbgk = nt;
     }
   }
   hipStream_t st = b->stream;
   if (!b->running.empty()) {
-    if (hipMemcpyAsync(b->d_doms, b->h_doms, b->running.size() * sizeof(BatchDomain), hipMemcpyHostToDevice, st) != hipSuccess)
+    if (hipMemcpyAsync(b->d_doms, h_doms, b->running.size() * sizeof(BatchDomain), hipMemcpyHostToDevice, st) !=
+        hipSuccess)
       return bfail(HLBM_ERR_HIP, "upload domains");
     if (b->tiles_dirty) {
-      b->h_ntiles[0] = nbgk;
-      b->h_ntiles[1] = nt;
-      if (hipMemcpyAsync(b->d_tiles, b->h_tiles, (size_t)nt * sizeof(BatchTile), hipMemcpyHostToDevice, st) != hipSuccess ||
-          hipMemcpyAsync(b->d_ntiles, b->h_ntiles, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, st) != hipSuccess)
+      h_ntiles[0] = nbgk;
+      h_ntiles[1] = nt;
+      if (hipMemcpyAsync(b->d_tiles, h_tiles, (size_t)nt * sizeof(BatchTile), hipMemcpyHostToDevice, st) != hipSuccess ||
+          hipMemcpyAsync(b->d_ntiles, h_ntiles, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, st) != hipSuccess)
         return bfail(HLBM_ERR_HIP, "upload tiles");
       b->tiles_dirty = false;
     }
     int steps_needed = 0;
     for (int64_t id : b->running) steps_needed = std::max(steps_needed, b->solves[id].dom.active_steps);
+    // the events also mark when this call's staging set is free again
+    auto& c = b->cev[set];
+    if (hipEventRecord(c.start, st) != hipSuccess) return bfail(HLBM_ERR_HIP, "event");
     if (steps_needed > 0) {
-      if (hipEventRecord(b->ev0, st) != hipSuccess) return bfail(HLBM_ERR_HIP, "event");
       if (b->graph && steps_needed == K) {
         if (hipGraphLaunch(b->graph, st) != hipSuccess) return bfail(HLBM_ERR_HIP, "graph launch");
       } else {
         for (int k = 0; k < steps_needed; k++)
           if (chunk_launch(b, k) != hipSuccess) return bfail(HLBM_ERR_HIP, "batch kernel launch");
       }
-      if (hipEventRecord(b->ev1, st) != hipSuccess || hipEventSynchronize(b->ev1) != hipSuccess)
-        return bfail(HLBM_ERR_HIP, "chunk sync");
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, b->ev0, b->ev1);
-      b->gpu_ms += ms;
-      b->chunks++;
     }
+    if (hipEventRecord(c.end, st) != hipSuccess) return bfail(HLBM_ERR_HIP, "event");
+    c.pending = true;
+    b->run_index++;
     for (int64_t id : b->running) {
       Solve& s = b->solves[id];
       const int a = s.dom.active_steps;
@@ -519,20 +619,24 @@ int hlbm_batch_run(hlbm_batch* b, int32_t* n_finished) {
       b->node_updates += (double)a * s.N;
     }
   }
-  // retirement
+  // retirement of the solves this chunk finishes: enqueued behind it
   std::vector<int64_t> keep;
   for (int64_t id : b->running) {
     Solve& s = b->solves[id];
     if (s.fused_left == 0) {
-      int rc = retire_one(b, s);
+      rc = retire_one(b, s);
       if (rc) return rc;
-      finished++;
       b->tiles_dirty = true;
     } else {
       keep.push_back(id);
     }
   }
   b->running.swap(keep);
+  // nothing left to run: wait for the outstanding read-outs instead of spinning
+  if (b->running.empty() && b->queue.empty() && !b->retiring.empty()) {
+    rc = poll_retiring(b, true, &finished);
+    if (rc) return rc;
+  }
   if (n_finished) *n_finished = finished;
   return HLBM_OK;
 }
@@ -541,6 +645,10 @@ int hlbm_batch_status(hlbm_batch* b, int64_t id, int32_t* state, int64_t* steps_
   if (!b) return bfail(HLBM_ERR_ARG, "NULL batch");
   auto it = b->solves.find(id);
   if (it == b->solves.end()) return bfail(HLBM_ERR_ARG, "unknown solve id");
+  if (it->second.retiring) {
+    const int rc = poll_retiring(b, false, nullptr);
+    if (rc) return rc;
+  }
   if (state) *state = it->second.state;
   if (steps_done) *steps_done = it->second.steps_done;
   return HLBM_OK;
@@ -562,18 +670,30 @@ int hlbm_batch_result(hlbm_batch* b, int64_t id, double* rho, double* u, double*
   if (it == b->solves.end()) return bfail(HLBM_ERR_ARG, "unknown solve id");
   Solve& s = it->second;
   if (s.state != HLBM_SOLVE_DONE) return bfail(HLBM_ERR_ARG, "solve not finished");
-  if (rho) std::memcpy(rho, s.rho.data(), s.rho.size() * 8);
-  if (u) std::memcpy(u, s.u.data(), s.u.size() * 8);
-  if (f) {
-    if (!s.desc.keep_populations) return bfail(HLBM_ERR_ARG, "populations were not kept (keep_populations = 0)");
-    std::memcpy(f, s.f.data(), s.f.size() * 8);
-  }
+  if (f && !s.desc.keep_populations) return bfail(HLBM_ERR_ARG, "populations were not kept (keep_populations = 0)");
+  // the read-out waits for nothing on the main stream: the fields are final
+  const double* dres = (const double*)((const char*)b->arena[s.res_arena] + s.f_off * b->esize);
+  if ((rho && hipMemcpyAsync(rho, dres, (size_t)s.N * 8, hipMemcpyDeviceToHost, b->copy_stream) != hipSuccess) ||
+      (u && hipMemcpyAsync(u, dres + s.N, 3 * (size_t)s.N * 8, hipMemcpyDeviceToHost, b->copy_stream) != hipSuccess) ||
+      hipStreamSynchronize(b->copy_stream) != hipSuccess)
+    return bfail(HLBM_ERR_HIP, "result copy");
+  if (f) std::memcpy(f, s.f.data(), s.f.size() * 8);
+  b->fl_pop.release(s.f_off, s.f_len);
+  b->fl_flag.release(s.flag_off, s.flag_len);
+  if (s.ev) (void)hipEventDestroy(s.ev);
   b->solves.erase(it);
   return HLBM_OK;
 }
 
 int hlbm_batch_info_get(hlbm_batch* b, hlbm_batch_info* out) {
   if (!b || !out) return bfail(HLBM_ERR_ARG, "NULL argument");
+  // a consistent snapshot: chunks in flight are waited for and accounted
+  for (int k = 0; k < 2; k++) {
+    const int rc = finish_chunk(b, (int)((b->run_index + k) & 1));
+    if (rc) return rc;
+  }
+  const int rc = poll_retiring(b, false, nullptr);
+  if (rc) return rc;
   out->queued = (int64_t)b->queue.size();
   out->running = (int64_t)b->running.size();
   out->done = b->n_done;

@@ -19,6 +19,16 @@ inline bool wide_offsets(const StepParams& p, size_t esize) {
   return (uint64_t)(p.N - 1u) * esize > 0xFFFFFFFFull;
 }
 
+// Uniform constants of the FP32-compute forms, derived from tau and c_i . F.
+inline void set_fp32_constants(StepParams& p) {
+  p.has_force = 0;
+  for (int i = 0; i < Q; i++) {
+    p.ef3f[i] = 3.0f * (float)p.ef[i];
+    if (p.ef[i] != 0.0) p.has_force = 1;
+  }
+  p.omega_f = (float)(1.0 / p.tau);
+}
+
 inline unsigned boundary_blocks(const StepParams& p) {
   uint64_t n = 0;
   for (int pl = 0; pl < 6; pl++)

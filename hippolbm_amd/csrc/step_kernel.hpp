@@ -90,6 +90,9 @@ struct StepParams {
   void* oflow;          // outflow values captured before an in-place AA step (C type), or null
   uint32_t seed;        // stochastic rounding: hash of the time step (0 with sr = 0)
   int32_t sr;           // 16-bit storage rounds stochastically
+  int32_t has_force;    // some c_i . F is non-zero (otherwise the force term adds exactly +0)
+  float omega_f;        // 1/tau for the FP32-compute forms
+  float ef3f[Q];        // 3 c_i . F for the FP32-compute forms
   double cav[2][5];
   double tau;
   double F[3];
@@ -266,21 +269,28 @@ __device__ __forceinline__ void bgk_f64(double* f, double rho, double ux, double
     const double w = WD(i);
     const double eu = cdot<i>(ux, uy, uz);
     const double feq = w * rho * (1. + 3. * eu + 4.5 * eu * eu - 1.5 * u_squ);
-    f[i] += (feq - f[i]) / tau + 3. * rho * w * p.ef[i];
+    // F = 0 makes the force term exactly +0 (c_i . F is evaluated to +0.0), so
+    // dropping it leaves every sum bit-identical
+    if (p.has_force)
+      f[i] += (feq - f[i]) / tau + 3. * rho * w * p.ef[i];
+    else
+      f[i] += (feq - f[i]) / tau;
   });
 }
 
 // BGK on shifted populations g = f - w, FP32.
-__device__ __forceinline__ void bgk_shift(float* g, float dr, float ux, float uy, float uz, const StepParams& p,
-                                          float omega, const float* ef3) {
+__device__ __forceinline__ void bgk_shift(float* g, float dr, float ux, float uy, float uz, const StepParams& p) {
   const float rho = 1.0f + dr;
+  const float omega = p.omega_f;
   const float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
+  const bool force = p.has_force;
   static_for([&](auto ic) {
     constexpr int i = decltype(ic)::value;
     const float w = WF(i);
     const float eu = cdot<i>(ux, uy, uz);
     const float geq = w * (dr + rho * (eu * (3.0f + 4.5f * eu) - usq15));
-    g[i] = g[i] + omega * (geq - g[i]) + rho * w * ef3[i];
+    g[i] = g[i] + omega * (geq - g[i]);
+    if (force) g[i] += rho * w * p.ef3f[i];
   });
 }
 
@@ -490,11 +500,13 @@ __device__ __forceinline__ void mrt_collide(C* f, C rho_force, const StepParams&
     constexpr int i = decltype(ic)::value;
     f[i] = inverse_row<i>(m);
   });
-  // mrt.hpp:187-191, force term with rho from macro_variables
-  static_for([&](auto ic) {
-    constexpr int i = decltype(ic)::value;
-    f[i] += C(3.) * rho_force * C(WD(i)) * C(p.ef[i]);
-  });
+  // mrt.hpp:187-191, force term with rho from macro_variables (exactly +0 when F = 0)
+  if (p.has_force) {
+    static_for([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      f[i] += C(3.) * rho_force * C(WD(i)) * C(p.ef[i]);
+    });
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -824,10 +836,7 @@ __device__ __forceinline__ void collide_node(const StepParams& p, uint32_t idx, 
       }
       if (!wall) {
         if constexpr (COLL == COLL_BGK) {
-          float ef3[Q];
-#pragma unroll
-          for (int i = 0; i < Q; i++) ef3[i] = 3.0f * (float)p.ef[i];
-          bgk_shift((float*)f, dr, ux, uy, uz, p, (float)(1.0 / p.tau), ef3);
+          bgk_shift((float*)f, dr, ux, uy, uz, p);
         } else {
           mrt_collide<float>((float*)f, rho, p, true);
         }
@@ -915,6 +924,16 @@ __device__ __forceinline__ T* scalar_ptr(T* p) {
   asm("" : "+s"(g));
   return (T*)g;
 }
+// A pointer the compiler cannot prove wave-uniform (loop-carried through
+// divergent control flow) but that is: read from the first active lane.
+template <class T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (T*)(((uint64_t)hi << 32) | lo);
+}
+
 // A per-lane byte offset defined anew where it is used, so the selector sees
 // (scalar base + zero-extended 32-bit offset) in the same block and emits the
 // scalar-base form of the access.
@@ -1263,6 +1282,9 @@ struct BatchDomain {
   uint64_t flag_off;     // byte offset in the flag arena
   double inv_nx;         // 1/nx and 1/(nx*ny): node coordinates without integer division
   double inv_nxy;
+  int32_t has_force;
+  float omega_f;
+  float ef3f[Q];
   double tau;
   double ef[Q];
   double cav[2][5];
@@ -1322,6 +1344,9 @@ __device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, c
     p.cav[1][k] = d.cav[1][k];
   }
   p.tau = d.tau;
+  p.has_force = d.has_force;
+  p.omega_f = d.omega_f;
+  for (int i = 0; i < Q; i++) p.ef3f[i] = d.ef3f[i];
   for (int i = 0; i < Q; i++) p.ef[i] = d.ef[i];
   for (int k = 0; k < 10; k++) p.mrt[k] = d.mrt[k];
   return p;
