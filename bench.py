@@ -126,9 +126,9 @@ def batched_cavities(n_dom=2048, chunk=200, seed=4, storage="f16"):
             "ms_per_step": ms / chunk, "frac": mlups * 1e6 * bpn / 1e9 / HBM_PEAK_GBS}
 
 
-def served_sweep(rates=(4.0, 16.0, 0.0), n=32, seed=5, storage="f16", chunk=200, budget_gb=96):
-    """served_stream at several arrival rates (0 = every request at t = 0)."""
-    return [served_stream(n, r, seed, storage, chunk, budget_gb) for r in rates]
+def served_sweep(rates=(4.0, 16.0, 64.0, 0.0), n=32, seed=5, storages=("f32", "f16"), chunk=200, budget_gb=96):
+    """served_stream at several arrival rates (0 = every request at t = 0), per storage."""
+    return {st: [served_stream(n, r, seed, st, chunk, budget_gb) for r in rates] for st in storages}
 
 
 def served_stream(n=32, rate_hz=4.0, seed=5, storage="f16", chunk=200, budget_gb=96):
@@ -289,9 +289,11 @@ def main():
         print(f"# batched cavities: {batched['mlups']:.0f} MLUPS ({100 * batched['frac']:.1f}%)", file=sys.stderr,
               flush=True)
         served = served_sweep()
-        for r in served:
-            print(f"# served @ {r['rate_hz']} Hz: p50 {r['p50_s']:.2f} s p99 {r['p99_s']:.2f} s, "
-                  f"{r['mlups']:.0f} MLUPS ({100 * r['frac']:.1f}%)", file=sys.stderr, flush=True)
+        for st, rs in served.items():
+            for r in rs:
+                print(f"# served {st} @ {r['rate_hz']} Hz: p50 {r['p50_s']:.2f} s p99 {r['p99_s']:.2f} s, "
+                      f"{r['mlups']:.0f} MLUPS ({100 * r['frac']:.1f}%), GPU busy {100 * r['gpu_busy']:.0f}%",
+                      file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline()
     line = {
         "metric": "MLUPS (D3Q19 BGK FP64 Couette 512^3, one MI355X) + % HBM roofline per storage dtype",
@@ -316,8 +318,9 @@ def main():
         "per_dtype": {k: {"mlups": v["mlups"], "frac": v["frac"], "achieved_gbs": v["achieved_gbs"],
                           "bytes_per_node": v["bytes_per_node"], "ms_per_step": v["ms_per_step"]}
                       for k, v in per.items()},
-        "latency": ({"p50_s": served[0]["p50_s"], "p99_s": served[0]["p99_s"],
-                     "workload": f"served stream, Poisson arrivals at {served[0]['rate_hz']} Hz"}
+        "latency": ({"p50_s": served["f32"][2]["p50_s"], "p99_s": served["f32"][2]["p99_s"],
+                     "aggregate_mlups": served["f32"][2]["mlups"], "frac": served["f32"][2]["frac"],
+                     "workload": f"served stream, FP32 storage, Poisson arrivals at {served['f32'][2]['rate_hz']} Hz"}
                     if served else None),
         "porous_channel": porous,
         "batched": batched,

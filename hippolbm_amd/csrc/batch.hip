@@ -347,6 +347,7 @@ int hlbm_batch_submit(hlbm_batch* b, const hlbm_solve_desc* s, int64_t* id) {
   for (int k = 0; k < 3; k++) d.per[k] = s->periodic[k] ? 1 : 0;
   d.coll = s->collision == HLBM_MRT ? COLL_MRT : COLL_BGK;
   d.wall_bb = s->wall_bb && s->n_obstacles > 0 ? 1 : 0;
+  d.any_wall = s->n_obstacles > 0 ? 1 : 0;
   d.mrt_custom = s->mrt_custom;
   d.inv_nx = 1.0 / (double)d.nx;
   d.inv_nxy = 1.0 / ((double)d.nx * (double)d.ny);
