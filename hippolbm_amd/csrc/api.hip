@@ -65,6 +65,7 @@ struct hlbm_domain {
   dim3 block{128, 2, 1};
   int xcd_remap = 1;
   int npt = 1;
+  int force_wide = 0;
   int device = 0;
   uint64_t bytes = 0;
 };
@@ -104,6 +105,8 @@ static StepParams base_params(hlbm_domain* h) {
   if (!h->any_wall) p.wall_bb = 0;
   p.any_wall = h->any_wall ? 1 : 0;
   p.bbm = h->bbm;
+  p.force_wide = h->force_wide;
+  p.pair32 = ((uint64_t)h->stride + h->N) * h->esize < 0xFFFFFFFFull ? 1 : 0;
   // 16-bit storage: stochastic rounding keyed on (node, time step)
   p.sr = (h->d.storage == HLBM_F16 || h->d.storage == HLBM_BF16) && !h->d.round_nearest ? 1 : 0;
   p.seed = mix32((uint32_t)h->t * 0x9E3779B9u + 1u);
@@ -224,6 +227,8 @@ int hlbm_domain_create(const hlbm_domain_desc* desc, hlbm_domain** out) {
   if (const char* pad = getenv("HLBM_PAD")) h->stride += (uint64_t)strtoull(pad, nullptr, 10);
   if (const char* xr = getenv("HLBM_XCD_REMAP")) h->xcd_remap = atoi(xr);
   if (const char* np = getenv("HLBM_NPT")) h->npt = atoi(np) == 2 ? 2 : 1;
+  // the 64-bit-address variant otherwise runs only on domains above 2^32 bytes per plane
+  if (const char* fw = getenv("HLBM_FORCE_WIDE")) h->force_wide = atoi(fw) ? 1 : 0;
   (void)hipGetDevice(&h->device);
   const char* blk = getenv("HLBM_BLOCK");
   if (blk) {

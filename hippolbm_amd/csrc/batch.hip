@@ -159,6 +159,7 @@ static StepParams single_params(hlbm_batch* b, const Solve& s, int parity) {
   p.flags = b->flags + s.flag_off;
   p.any_wall = 1;
   p.bbm = b->bbm + s.flag_off;  // built at admission
+  p.pair32 = 1;                 // checked at submission
   p.sr = b->sr;
   p.seed = mix32((uint32_t)s.steps_done * 0x9E3779B9u + 1u);
   p.wall_bb = s.dom.wall_bb;

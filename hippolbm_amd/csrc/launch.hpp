@@ -15,6 +15,7 @@ namespace hlbm {
 // some offset, (N-1)*element size, needs more than 32 bits, or when wall
 // bounce-back is active and no mask was supplied.
 inline bool wide_offsets(const StepParams& p, size_t esize) {
+  if (p.force_wide) return true;
   if (p.any_wall && p.wall_bb && !p.bbm) return true;
   return (uint64_t)(p.N - 1u) * esize > 0xFFFFFFFFull;
 }

@@ -91,6 +91,8 @@ struct StepParams {
   uint32_t seed;        // stochastic rounding: hash of the time step (0 with sr = 0)
   int32_t sr;           // 16-bit storage rounds stochastically
   int32_t has_force;    // some c_i . F is non-zero (otherwise the force term adds exactly +0)
+  int32_t force_wide;   // take the 64-bit-address bulk variant even when 32-bit offsets would do
+  int32_t pair32;       // (stride + N) * element size < 2^32: bounce-back waves gather without branches
   float omega_f;        // 1/tau for the FP32-compute forms
   float ef3f[Q];        // 3 c_i . F for the FP32-compute forms
   double cav[2][5];
@@ -999,6 +1001,40 @@ __device__ __forceinline__ bool nbr_outside(const NodeNbrs& a, bool& zface) {
   return o;
 }
 
+// Gather without branches (same result as gather_node), for waves that hold
+// bounce-back lanes: the opposite directions (2k-1, 2k) share one scalar base,
+// their first plane, so a lane that bounces back (face or wall upstream) loads
+// f*_opp(i)(x) in the same instruction as the other lanes' pull, by selecting
+// the other offset.  Needs (stride + N) * sizeof(S) < 2^32 (p.pair32).
+template <class S, int MODE>
+__device__ __forceinline__ void gather_select(const StepParams& p, uint32_t bo, const NodeNbrs& a, uint32_t wbits,
+                                              typename StoreOps<S>::C* f) {
+  using O = StoreOps<S>;
+  using C = typename O::C;
+  constexpr bool AA = mode_aa(MODE);
+  const S* __restrict__ fin = (const S*)p.fin;
+  const uint32_t stb = (uint32_t)(p.stride * sizeof(S));
+  bo = lane_offset(bo);
+  f[0] = O::ld(at_bytes(fin, bo), 0);
+  const S* b = fin;
+  static_for<1, Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    constexpr int lo = (i & 1) ? i : i - 1;  // first plane of the pair {i, opp(i)}
+    if constexpr (i & 1) b = scalar_ptr(b + (size_t)(i == 1 ? 1 : 2) * p.stride);
+    bool zf;
+    const bool bb = nbr_outside<i, true>(a, zf) || ((wbits >> i) & 1u);
+    // A/B: the pull reads plane i, the bounce-back plane opp(i); AA (swapped): the reverse
+    const uint32_t pull = ((AA ? OPP(i) : i) - lo) * stb + bo + nbr_offset<i, true>(a);
+    const uint32_t back = ((AA ? i : OPP(i)) - lo) * stb + bo;
+    C v = O::ld(at_bytes(b, bb ? back : pull), 0);
+    if constexpr (CZ(i) != 0) {
+      constexpr int side = CZ(i) == 1 ? 0 : 1;
+      if (p.cav_gather && p.cav_on[side]) v += (zf ? C(p.cav[side][cavity_slot(i)]) : C(0));
+    }
+    f[i] = v;
+  });
+}
+
 // Gather for a bulk node (same result as gather_node): all 19 pulls are issued
 // with wrapped offsets, then, only in waves holding a bounce-back lane, the
 // lanes whose upstream node is outside a non-periodic face or (wall_bb) a wall
@@ -1020,6 +1056,11 @@ __device__ __forceinline__ void gather_bulk(const StepParams& p, uint32_t bo, co
     });
   } else {
     constexpr bool AA = mode_aa(MODE);
+    const bool edge = a.fm[0] || a.fp[0] || a.fm[1] || a.fp[1] || a.fm[2] || a.fp[2] || wbits > 1u;
+    if (p.pair32 && __any(edge)) {
+      gather_select<S, MODE>(p, bo, a, wbits, f);
+      return;
+    }
     // plane j holds direction i = j (A/B) or i = opp(j) (AA, swapped)
     static_for([&](auto jc) {
       constexpr int j = decltype(jc)::value;
@@ -1027,7 +1068,6 @@ __device__ __forceinline__ void gather_bulk(const StepParams& p, uint32_t bo, co
       b = scalar_ptr(j ? b + st : b);
       f[i] = O::ld(at_bytes(b, bo + nbr_offset<i, true>(a)), 0);
     });
-    const bool edge = a.fm[0] || a.fp[0] || a.fm[1] || a.fp[1] || a.fm[2] || a.fp[2] || wbits > 1u;
     if (__any(edge)) {
       static_for<1, Q>([&](auto ic) {
         constexpr int i = decltype(ic)::value;
@@ -1339,6 +1379,8 @@ __device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, c
   p.prog_planes = 0;
   p.oflow = nullptr;
   p.oflow_n = 0;
+  p.pair32 = 1;  // checked when the solve was submitted
+  p.force_wide = 0;
   p.sr = bp.sr;
   p.seed = mix32((d.t0 + (uint32_t)bp.step) * 0x9E3779B9u + 1u);
   p.any_wall = 1;
@@ -1355,38 +1397,6 @@ __device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, c
   for (int i = 0; i < Q; i++) p.ef[i] = d.ef[i];
   for (int k = 0; k < 10; k++) p.mrt[k] = d.mrt[k];
   return p;
-}
-
-// Gather for a batch node (same result as gather_node), without branches: the
-// opposite directions (2k-1, 2k) share one scalar base (their first plane), so
-// a lane that bounces back (face or wall upstream) loads f*_opp(i)(x) in the
-// same instruction as its neighbours' pull, by selecting the other offset.
-// Needs (stride + N) * sizeof(S) < 2^32 (checked when a solve is submitted).
-template <class S>
-__device__ __forceinline__ void gather_select(const StepParams& p, uint32_t bo, const NodeNbrs& a, uint32_t wbits,
-                                              typename StoreOps<S>::C* f) {
-  using O = StoreOps<S>;
-  using C = typename O::C;
-  const S* __restrict__ fin = (const S*)p.fin;
-  const uint32_t stb = (uint32_t)(p.stride * sizeof(S));
-  bo = lane_offset(bo);
-  f[0] = O::ld(at_bytes(fin, bo), 0);
-  const S* b = fin;
-  static_for<1, Q>([&](auto ic) {
-    constexpr int i = decltype(ic)::value;
-    constexpr int lo = (i & 1) ? i : i - 1;  // first plane of the pair {i, opp(i)}
-    if constexpr (i & 1) b = scalar_ptr(b + (size_t)(i == 1 ? 1 : 2) * p.stride);
-    bool zf;
-    const bool bb = nbr_outside<i, true>(a, zf) || ((wbits >> i) & 1u);
-    const uint32_t pull = (i - lo) * stb + bo + nbr_offset<i, true>(a);
-    const uint32_t back = (OPP(i) - lo) * stb + bo;
-    C v = O::ld(at_bytes(b, bb ? back : pull), 0);
-    if constexpr (CZ(i) != 0) {
-      constexpr int side = CZ(i) == 1 ? 0 : 1;
-      if (p.cav_on[side]) v += (zf ? C(p.cav[side][cavity_slot(i)]) : C(0));
-    }
-    f[i] = v;
-  });
 }
 
 // One launch per collision model and time step: the BGK launch walks the
@@ -1430,7 +1440,7 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchParams bp) {
     const uint32_t bo = n * (uint32_t)sizeof(S);
     const NodeNbrs a = node_nbrs<S>(p, x, y, z);
     C f[Q];
-    gather_select<S>(p, bo, a, wbits, f);
+    gather_select<S, MODE_FUSED>(p, bo, a, wbits, f);
     collide_node<S, COLL, MODE_FUSED>(p, n, wall, f);
     store_bulk<S, MODE_FUSED>(p, bo, a, wbits, f);
   }

@@ -252,6 +252,23 @@ def test_porous_channel_open_boundaries_bitwise(collision, outlet, pattern):
     sim.dom.close()
 
 
+@pytest.mark.parametrize("pattern", ["ab", "aa"])
+def test_wide_address_variant_bitwise(pattern, monkeypatch):
+    # the 64-bit-address bulk variant (domains above 2^32 bytes per population
+    # plane) forced onto a small porous channel: the same populations
+    monkeypatch.setenv("HLBM_FORCE_WIDE", "1")
+    d = _porous_deck(30, 20, 18, 6, 4, "bgk")
+    say("wide", pattern)
+    ref = orc.OracleRun(d)
+    sim = Simulation(d, storage="f64", pattern=pattern)
+    monkeypatch.delenv("HLBM_FORCE_WIDE")
+    ref.step(23)
+    sim.run(23)
+    nbad, maxdiff, _ = compare(sim, ref, fluid_only=True)
+    assert nbad == 0, (nbad, maxdiff)
+    sim.dom.close()
+
+
 def test_outflow_after_an_operator_on_its_interior_layer_is_rejected():
     d = _channel_deck(20, 10, 10, 0, 1, "bgk")
     d.boundary_conditions.insert(0, ("lid_driven_cavity", {"U": (0.01, 0.0, 0.0), "regions": ["plan_xy_l"]}))
