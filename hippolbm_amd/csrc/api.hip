@@ -752,6 +752,42 @@ int hlbm_reduce_stats(hlbm_domain* h, double out[7]) {
   return HLBM_OK;
 }
 
+int hlbm_plane_profile(hlbm_domain* h, int32_t dim, double scale, double* sum, int32_t* count, double* vmin,
+                       double* vmax) {
+  if (!h || !sum || !count || !vmin || !vmax) return fail(HLBM_ERR_ARG, "NULL argument");
+  if (dim < 0 || dim > 2) return fail(HLBM_ERR_ARG, "dim must be 0 (x), 1 (y) or 2 (z)");
+  const int n[3] = {h->nx, h->ny, h->nz};
+  const size_t parts = profile_parts(h->nx, h->ny, h->nz, dim);
+  ProfileStat* dev = nullptr;
+  HIP_TRY(hipMalloc((void**)&dev, (parts + (size_t)n[dim]) * sizeof(ProfileStat)));
+  std::vector<ProfileStat> host(n[dim]);
+  hipError_t e = launch_plane_profile(h->m1, h->flags, h->N, h->nx, h->ny, h->nz, dim, scale, dev, dev + parts, 0);
+  if (e == hipSuccess)
+    e = hipMemcpy(host.data(), dev + parts, (size_t)n[dim] * sizeof(ProfileStat), hipMemcpyDeviceToHost);
+  (void)hipFree(dev);
+  if (e != hipSuccess) return fail(HLBM_ERR_HIP, hipGetErrorString(e));
+  for (int p = 0; p < n[dim]; p++) {
+    sum[p] = host[p].sum;
+    count[p] = host[p].count;
+    vmin[p] = host[p].vmin;
+    vmax[p] = host[p].vmax;
+  }
+  return HLBM_OK;
+}
+
+int hlbm_line_velocity(hlbm_domain* h, const int32_t start[3], int32_t axis, int32_t n, double scale, double* u) {
+  if (!h || !start || !u) return fail(HLBM_ERR_ARG, "NULL argument");
+  if (axis < 0 || axis > 2 || n < 1) return fail(HLBM_ERR_ARG, "axis must be 0..2 and n >= 1");
+  double* dev = nullptr;
+  HIP_TRY(hipMalloc((void**)&dev, (size_t)n * 3 * 8));
+  const int st[3] = {start[0], start[1], start[2]};
+  hipError_t e = launch_line_velocity(h->m1, h->N, h->nx, h->ny, h->nz, st, axis, n, scale, dev, 0);
+  if (e == hipSuccess) e = hipMemcpy(u, dev, (size_t)n * 3 * 8, hipMemcpyDeviceToHost);
+  (void)hipFree(dev);
+  if (e != hipSuccess) return fail(HLBM_ERR_HIP, hipGetErrorString(e));
+  return HLBM_OK;
+}
+
 int hlbm_set_collision(hlbm_domain* h, int32_t collision, const double* mrt_rates) {
   if (!h) return fail(HLBM_ERR_ARG, "NULL domain");
   if (collision != HLBM_BGK && collision != HLBM_MRT) return fail(HLBM_ERR_ARG, "collision must be BGK or MRT");

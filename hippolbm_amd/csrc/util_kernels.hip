@@ -196,6 +196,109 @@ __global__ void k_reduce_final(const double* part, int nparts, double* out) {
   for (int k = 0; k < 7; k++) out[k] = r[k];
 }
 
+// plane_velocity_profile (src/check/plane_velocity_profile.cu:81-101): per
+// plane perpendicular to `dim`, over the fluid nodes, |scale*u| summed, counted,
+// min and max.  Pass 1: one thread per (x, a) column walks the other axis b,
+// where (a, b) = (z, y) for dim x or z and (y, z) for dim y, so every thread
+// feeds a single plane and every row of a wave is one coalesced read; it
+// writes the column's statistics to part[a*nx + x].  Pass 2: one block per
+// plane folds its columns in a fixed order (deterministic sums).
+__global__ void __launch_bounds__(256)
+    k_profile_cols(const double* m1, const uint8_t* flags, uint32_t N, int nx, int ny, int nz, int dim,
+                   double scale, ProfileStat* part) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = blockIdx.y;
+  if (x >= nx) return;
+  const size_t nxy = (size_t)nx * ny;
+  size_t idx, step;
+  int nb;
+  if (dim == 1) {  // a = y, b = z
+    idx = (size_t)a * nx + x;
+    step = nxy;
+    nb = nz;
+  } else {  // a = z, b = y
+    idx = (size_t)a * nxy + x;
+    step = nx;
+    nb = ny;
+  }
+  double sum = 0.0, vmin = 1.7976931348623157e308, vmax = -1.7976931348623157e308;
+  int cnt = 0;
+  for (int b = 0; b < nb; b++, idx += step) {
+    if (flags[idx]) continue;
+    const double ux = scale * m1[idx], uy = scale * m1[(size_t)N + idx], uz = scale * m1[2 * (size_t)N + idx];
+    const double v = sqrt(ux * ux + uy * uy + uz * uz);
+    sum += v;
+    cnt += 1;
+    vmin = fmin(vmin, v);
+    vmax = fmax(vmax, v);
+  }
+  ProfileStat o;
+  o.sum = sum;
+  o.vmin = vmin;
+  o.vmax = vmax;
+  o.count = cnt;
+  o.pad = 0;
+  part[(size_t)a * nx + x] = o;
+}
+
+__global__ void __launch_bounds__(256)
+    k_profile_planes(const ProfileStat* part, int nx, int na, int dim, ProfileStat* out) {
+  __shared__ double s_sum[256], s_min[256], s_max[256];
+  __shared__ int s_cnt[256];
+  const int p = blockIdx.x, t = threadIdx.x;
+  // dim x: plane p = column x of every a; dims y/z: plane p = row a of every x
+  const int n = dim == 0 ? na : nx;
+  const size_t first = dim == 0 ? (size_t)p : (size_t)p * nx;
+  const size_t step = dim == 0 ? (size_t)nx : 1;
+  double sum = 0.0, vmin = 1.7976931348623157e308, vmax = -1.7976931348623157e308;
+  int cnt = 0;
+  for (int k = t; k < n; k += 256) {
+    const ProfileStat c = part[first + (size_t)k * step];
+    sum += c.sum;
+    cnt += c.count;
+    vmin = fmin(vmin, c.vmin);
+    vmax = fmax(vmax, c.vmax);
+  }
+  s_sum[t] = sum;
+  s_min[t] = vmin;
+  s_max[t] = vmax;
+  s_cnt[t] = cnt;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      s_sum[t] += s_sum[t + w];
+      s_min[t] = fmin(s_min[t], s_min[t + w]);
+      s_max[t] = fmax(s_max[t], s_max[t + w]);
+      s_cnt[t] += s_cnt[t + w];
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    ProfileStat o;
+    o.sum = s_sum[0];
+    o.vmin = s_min[0];
+    o.vmax = s_max[0];
+    o.count = s_cnt[0];
+    o.pad = 0;
+    out[p] = o;
+  }
+}
+
+// plot_line_velocity (src/check/plot_over_line.cu:77-90): scale*u at the
+// nodes start + k*e_axis; nodes outside the domain give 0.
+__global__ void k_line_velocity(const double* m1, uint32_t N, int nx, int ny, int nz, int x0, int y0, int z0,
+                                int axis, int n, double scale, double* out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int x = x0 + (axis == 0 ? k : 0), y = y0 + (axis == 1 ? k : 0), z = z0 + (axis == 2 ? k : 0);
+  double u[3] = {0.0, 0.0, 0.0};
+  if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz) {
+    const size_t idx = ((size_t)z * ny + y) * nx + x;
+    for (int c = 0; c < 3; c++) u[c] = scale * m1[(size_t)c * N + idx];
+  }
+  for (int c = 0; c < 3; c++) out[3 * (size_t)k + c] = u[c];
+}
+
 // Streaming copy (16-byte vectors), the HBM reference point for the roofline
 // and for scaling the FETCH_SIZE / WRITE_SIZE counters.
 __global__ void __launch_bounds__(256) hbm_copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src,
@@ -341,6 +444,28 @@ hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s
 int reduce_parts(uint32_t N) {
   int parts = (int)((N + RED_BLOCK - 1) / RED_BLOCK);
   return parts > 1024 ? 1024 : (parts < 1 ? 1 : parts);
+}
+
+size_t profile_parts(int nx, int ny, int nz, int dim) { return (size_t)nx * (dim == 1 ? ny : nz); }
+
+hipError_t launch_plane_profile(const double* m1, const uint8_t* flags, uint32_t N, int nx, int ny, int nz, int dim,
+                                double scale, ProfileStat* part, ProfileStat* out, hipStream_t s) {
+  const int na = dim == 1 ? ny : nz;
+  hipLaunchKernelGGL(k_profile_cols, dim3((unsigned)((nx + 255) / 256), (unsigned)na), dim3(256), 0, s, m1, flags, N,
+                     nx, ny, nz, dim, scale, part);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const int planes = dim == 0 ? nx : na;
+  hipLaunchKernelGGL(k_profile_planes, dim3((unsigned)planes), dim3(256), 0, s, (const ProfileStat*)part, nx, na, dim,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t launch_line_velocity(const double* m1, uint32_t N, int nx, int ny, int nz, const int* start, int axis,
+                                int n, double scale, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_line_velocity, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m1, N, nx, ny, nz, start[0],
+                     start[1], start[2], axis, n, scale, out);
+  return hipGetLastError();
 }
 
 hipError_t launch_reduce(const double* m0, const double* m1, const uint8_t* flags, uint32_t N, double* part,

@@ -17,6 +17,12 @@ struct ObstacleDev {
   double r2;      // ball radius squared
 };
 
+// per-plane statistics of plane_velocity_profile
+struct ProfileStat {
+  double sum, vmin, vmax;
+  int32_t count, pad;
+};
+
 hipError_t launch_rasterize(uint8_t* flags, int nx, int ny, int nz, double dx, const ObstacleDev* obs, int nobs,
                             hipStream_t s);
 hipError_t launch_build_bbm(const uint8_t* flags, int nx, int ny, int nz, const int* per, uint32_t* bbm,
@@ -30,5 +36,12 @@ hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s
 int reduce_parts(uint32_t N);
 hipError_t launch_reduce(const double* m0, const double* m1, const uint8_t* flags, uint32_t N, double* part,
                          double* out, hipStream_t s);
+
+// column partials needed by launch_plane_profile (ProfileStat entries)
+size_t profile_parts(int nx, int ny, int nz, int dim);
+hipError_t launch_plane_profile(const double* m1, const uint8_t* flags, uint32_t N, int nx, int ny, int nz, int dim,
+                                double scale, ProfileStat* part, ProfileStat* out, hipStream_t s);
+hipError_t launch_line_velocity(const double* m1, uint32_t N, int nx, int ny, int nz, const int* start, int axis,
+                                int n, double scale, double* out, hipStream_t s);
 
 }  // namespace hlbm

@@ -200,6 +200,26 @@ class Domain:
         check("hlbm_get_macro", lib().hlbm_get_macro(self.h, _dp(rho), _dp(u), layout))
         return rho, u
 
+    def plane_profile(self, dim, scale=1.0):
+        """plane_velocity_profile statistics of |scale*u| per plane perpendicular to
+        axis `dim` (0, 1, 2) over the fluid nodes of the held u field:
+        (sum, count, min, max) arrays."""
+        n = self.nodes[dim]
+        s, lo, hi = np.zeros(n), np.zeros(n), np.zeros(n)
+        c = np.zeros(n, dtype=np.int32)
+        check("hlbm_plane_profile", lib().hlbm_plane_profile(self.h, int(dim), float(scale), _dp(s),
+                                                             c.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                             _dp(lo), _dp(hi)))
+        return s, c, lo, hi
+
+    def line_velocity(self, start, axis, n, scale=1.0):
+        """plot_line_velocity samples: scale*u at start + k*e_axis, shape (n, 3)."""
+        st = np.asarray(start, dtype=np.int32)
+        u = np.zeros((int(n), 3))
+        check("hlbm_line_velocity", lib().hlbm_line_velocity(self.h, st.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                                             int(axis), int(n), float(scale), _dp(u)))
+        return u
+
     def stats(self):
         """simulation_state over the rho/u fields: sum rho, min/max |u|, momentum, fluid count."""
         out = np.zeros(7)

@@ -4,7 +4,11 @@
 // regression checker fails, as the reference does.
 //
 //   hlbm_run deck.msp [--storage f64|f32|f16|bf16] [--pattern ab|aa] [--autotune]
-//                     [--steps N] [--quiet] [--dump-deck]
+//                     [--steps N] [--quiet] [--no-output] [--dump-deck]
+//
+// Output operators (write_paraview, plane_velocity_profile,
+// plot_line_velocity) write under the deck's output_directory, relative to
+// the working directory, as the reference does; --no-output turns them off.
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -15,12 +19,12 @@
 int main(int argc, char** argv) {
   if (argc < 2) {
     std::fprintf(stderr, "usage: %s deck.msp [--storage f64|f32|f16|bf16] [--pattern ab|aa] [--autotune] "
-                         "[--steps N] [--quiet] [--dump-deck]\n", argv[0]);
+                         "[--steps N] [--quiet] [--no-output] [--dump-deck]\n", argv[0]);
     return 2;
   }
   std::string path;
   std::string storage = "f64", pattern = "ab";
-  bool autotune = false, quiet = false, dump = false;
+  bool autotune = false, quiet = false, dump = false, io = true;
   long steps = -1;
   for (int i = 1; i < argc; i++) {
     const std::string a = argv[i];
@@ -29,6 +33,7 @@ int main(int argc, char** argv) {
     else if (a == "--steps" && i + 1 < argc) steps = std::atol(argv[++i]);
     else if (a == "--autotune") autotune = true;
     else if (a == "--quiet") quiet = true;
+    else if (a == "--no-output") io = false;
     else if (a == "--dump-deck") dump = true;
     else if (path.empty()) path = a;
     else {
@@ -54,9 +59,11 @@ int main(int argc, char** argv) {
     else throw hlbm_host::OperatorError("unknown pattern " + pattern);
     s.autotune = autotune;
     s.quiet = quiet;
+    s.io = io;
     if (steps >= 0) s.end_iteration = steps;
     sim.initialize();
     sim.compute_loop();
+    sim.final_output();
     sim.checker();
     if (quiet)
       std::printf("final step %ld sum_density %.17g min_velocity_norm %.17g max_velocity_norm %.17g\n",

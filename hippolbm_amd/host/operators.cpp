@@ -118,6 +118,7 @@ struct LbmParametersOp : OperatorNode {
     }
     P.celerity = c_;
     P.nuth = nuth;
+    if (in.count("avg_rho")) P.avg_rho = in["avg_rho"];
     const double dxc = c_ * P.dtLB;
     for (int d = 0; d < 3; d++) P.Fext[d] = F[d] * P.dtLB * P.dtLB / dxc;
     s.has_params = true;
@@ -166,6 +167,7 @@ struct SetDistribution : OperatorNode {
   }
   void execute(Slots& s) override {
     check(hlbm_set_distribution(s.dom, value, has_bounds ? b : nullptr), "set_distribution");
+    if (!has_bounds) s.set_dist_value = value;
   }
 };
 
@@ -509,11 +511,21 @@ Simulation::Simulation(const msp::Node& deck) {
   set_distributions = make_list(deck.find("set_distributions"));
   set_obstacles = make_list(deck.find("set_obstacles"));
   checker_op = make_list(deck.find("checker_op"));
+  analysis = make_list(deck.find("analysis"));
+  if (const msp::Node* c = deck.find("checker")) {
+    checker_list = make_list(c);
+    checker_override = true;
+  }
+  paraview = OperatorNodeFactory::instance()->make("write_paraview", msp::Node{});
   if (const msp::Node* g = deck.find("global")) {
     slots.end_iteration = (long)num_or(*g, "simulation_end_iteration", 100);
     slots.print_log_freq = (long)num_or(*g, "simulation_print_log_freq", 1);
     slots.timestep = (long)num_or(*g, "timestep", 0);
     slots.physical_time = num_or(*g, "physical_time", 0.0);
+    slots.paraview_freq = (long)num_or(*g, "simulation_paraview_freq", -1);
+    slots.analysis_freq = (long)num_or(*g, "simulation_analysis_freq", -1);
+    if (const msp::Node* n = g->find("display_paraview_first_timestep")) slots.display_paraview_first_timestep = n->boolean();
+    if (const msp::Node* n = g->find("output_directory")) slots.output_directory = n->text;
   }
   // extension (mrt.hpp hard-codes its rates): mrt_rates: [s1 s2 s4 s9 s10 s13 s16 weps wepsj wxx]
   if (const msp::Node* r = deck.find("mrt_rates")) {
@@ -546,8 +558,13 @@ void Simulation::initialize() {
   UpdateObstacles().execute(s);
   MacroVariables().execute(s);
   SimulationState().execute(s);
+  if (s.display_paraview_first_timestep) {  // write_paraview_first_timestep (config_io.msp:20-27)
+    MacroVariables().execute(s);
+    paraview->execute(s);
+  }
   s.previous_time = std::chrono::steady_clock::now();
   if (s.print_log_freq > 0) Log().execute(s);
+  run(analysis, s);
   s.timestep += 1;  // next_time_step
   s.physical_time += s.params.dtLB;
 }
@@ -579,11 +596,19 @@ void Simulation::compute_loop() {
     s.recording.clear();
     s.streamed = false;
     s.pending++;
-    // print_log_if_triggered (config_io.msp: simulation_state + log)
+    // end_iteration (config_compute_loop.msp:8-16): dump_paraview_if_triggered
+    // (update_ghost + macro_variables + write_paraview), print_log_if_triggered
+    // (simulation_state + log), print_analysis_if_triggered
+    if (s.paraview_freq > 0 && s.timestep % s.paraview_freq == 0 && s.io) {
+      s.flush();
+      check(hlbm_macro(s.dom, nullptr, nullptr, HLBM_LAYOUT_NODES), "macro_variables");
+      paraview->execute(s);
+    }
     if (s.print_log_freq > 0 && s.timestep % s.print_log_freq == 0) {
       SimulationState().execute(s);
       Log().execute(s);
     }
+    if (s.analysis_freq > 0 && s.timestep % s.analysis_freq == 0) run(analysis, s);
     s.timestep += 1;  // next_time_step
     s.physical_time += s.params.dtLB;
   }
@@ -591,7 +616,13 @@ void Simulation::compute_loop() {
   s.in_loop = false;
 }
 
+void Simulation::final_output() { paraview->execute(slots); }
+
 void Simulation::checker() {
+  if (checker_override) {
+    run(checker_list, slots);
+    return;
+  }
   SimulationState().execute(slots);
   run(checker_op, slots);
 }

@@ -34,6 +34,7 @@ namespace hlbm_host {
 // LBMParameters (src/grid/include/hippoLBM/grid/lbm_parameters.hpp:41-142)
 struct LBMParameters {
   double tau = 0.0, nu = 0.0, dtLB = 0.0, celerity = 1.0, nuth = 1e-4;
+  double avg_rho = 1000.0;  // physical units (lbm_parameters.cpp:52)
   double Fext[3] = {0.0, 0.0, 0.0};  // LBM units
 };
 
@@ -66,6 +67,12 @@ struct Slots {
   int32_t storage = HLBM_F64;
   int32_t pattern = HLBM_PATTERN_AB;
   bool autotune = false;
+  bool io = true;  // false: write_paraview / profile / line operators write nothing
+  // output (global block)
+  std::string output_directory = "hippoLBMOutputDir";
+  long paraview_freq = -1, analysis_freq = -1;
+  bool display_paraview_first_timestep = true;
+  double set_dist_value = 1.0;  // value of the last set_distribution without bounds
   // grid / fields
   hlbm_domain* dom = nullptr;
   std::map<long, hlbm_obstacle> obstacles;  // register_solid_*: by id
@@ -126,11 +133,14 @@ std::vector<std::unique_ptr<OperatorNode>> make_list(const msp::Node* list);
 struct Simulation {
   Slots slots;
   std::vector<std::unique_ptr<OperatorNode>> do_domain, set_lbm_parameters, collision, pre_stream_bcs,
-      post_stream_bcs, boundary_conditions, set_distributions, set_obstacles, checker_op;
+      post_stream_bcs, boundary_conditions, set_distributions, set_obstacles, checker_op, analysis, checker_list;
+  bool checker_override = false;  // the deck redefines `checker`
+  std::unique_ptr<OperatorNode> paraview;  // write_paraview with the default slots
 
   explicit Simulation(const msp::Node& deck);
   void initialize();    // config_initialize.msp:14-32
   void compute_loop();  // config_compute_loop.msp:23-38
+  void final_output();  // the write_paraview that follows compute_loop (main-config.msp:31)
   void checker();       // config_verification.msp
 };
 

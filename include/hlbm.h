@@ -186,6 +186,20 @@ int hlbm_get_macro(hlbm_domain* h, double* rho, double* u, int32_t layout);
  * out[3..5] = sum rho*u (momentum), out[6] = number of fluid nodes. */
 int hlbm_reduce_stats(hlbm_domain* h, double out[7]);
 
+/* plane_velocity_profile (src/check/plane_velocity_profile.cu:81-101,175-205):
+ * for each plane perpendicular to dim (0 x, 1 y, 2 z), over the fluid nodes
+ * of the domain's u field as it is held (the m1_ slot), v = |scale*u|:
+ * sum[p], count[p], vmin[p], vmax[p], one entry per node plane (nodes[dim]
+ * entries).  A plane without fluid gives count 0, vmin DBL_MAX, vmax -DBL_MAX,
+ * as the reference's reset values.  Sums are folded on the device in a fixed
+ * order (deterministic); the reference adds with atomics. */
+int hlbm_plane_profile(hlbm_domain* h, int32_t dim, double scale, double* sum, int32_t* count, double* vmin,
+                       double* vmax);
+/* plot_line_velocity (src/check/plot_over_line.cu:77-90,163-172): scale*u of
+ * the domain's u field at the n nodes start + k*e_axis (node coordinates,
+ * 0-based), written u[3k + c]; nodes outside the domain give 0. */
+int hlbm_line_velocity(hlbm_domain* h, const int32_t start[3], int32_t axis, int32_t n, double scale, double* u);
+
 /* update_ghost (src/grid/update_ghost.cu:66-73): on one GPU there is no halo;
  * this fills the periodic ghost layers of a reference-layout FP64 population
  * array from its Real nodes, on the host array given.  It exists so an Onika

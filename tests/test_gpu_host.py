@@ -22,10 +22,11 @@ RUN = os.path.join(ROOT, "hippolbm_amd", "hlbm_run")
 DECK_FILES = sorted(glob.glob(os.path.join(DECKS, "regression_*.msp")))
 
 
-def run(args):
+def run(args, cwd):
+    # output operators write under the working directory, as the reference does
     print("hlbm_run", *args, flush=True)
     sys.stdout.flush()
-    return subprocess.run([RUN] + args, capture_output=True, text=True, timeout=120)
+    return subprocess.run([RUN] + args, capture_output=True, text=True, timeout=120, cwd=cwd)
 
 
 def final_state(out):
@@ -34,10 +35,15 @@ def final_state(out):
 
 
 @pytest.mark.parametrize("path", DECK_FILES, ids=[os.path.basename(p) for p in DECK_FILES])
-def test_regression_deck_passes_its_checker(path):
-    r = run([path, "--quiet"])
+def test_regression_deck_passes_its_checker(path, tmp_path):
+    r = run([path, "--quiet"], tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "Test Pass." in r.stdout
+    # the first-timestep and final paraview dumps of main-config.msp
+    pv = tmp_path / "hippoLBMOutputDir" / "ParaviewOutput"
+    d0 = dk.load_msp(path)
+    assert (pv / "hippoLBM_0000000000" / "000000.vtr").exists()
+    assert (pv / ("hippoLBM_%010d.pvtr" % (d0.end_iteration + 1))).exists()
     step, s, vmin, vmax = final_state(r.stdout)
     d = dk.load_msp(path)
     ref = orc.OracleRun(d)
@@ -49,9 +55,9 @@ def test_regression_deck_passes_its_checker(path):
     assert abs(vmax - o[2]) <= 1e-12 * abs(o[2])
 
 
-def test_log_table_and_aa_pattern():
+def test_log_table_and_aa_pattern(tmp_path):
     path = os.path.join(DECKS, "regression_lbm_cavite_wall.msp")
-    r = run([path, "--pattern", "aa", "--steps", "7"])
+    r = run([path, "--pattern", "aa", "--steps", "7", "--no-output"], tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     lines = r.stdout.splitlines()
     assert lines[0].split() == ["Step", "Time", "Mesh", "Size", "Sum(density)", "min(||V||)", "max(||V||)", "MLUPS"]
@@ -64,6 +70,6 @@ def test_failing_checker_exits_with_failure(tmp_path):
     txt = open(os.path.join(DECKS, "regression_lbm_cavite.msp")).read().replace("density: 29791", "density: 29000")
     deck = tmp_path / "bad_density.msp"
     deck.write_text(txt)
-    r = run([str(deck), "--quiet"])
+    r = run([str(deck), "--quiet"], tmp_path)
     assert r.returncode != 0
     assert "Regression test checker [macro quantities]" in r.stdout and "Sum of densities" in r.stdout

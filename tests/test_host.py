@@ -1,6 +1,7 @@
 """The C++ operator layer's deck reader (no GPU): hlbm_run --dump-deck renders
 every regression deck exactly as a YAML loader reads it, and a deck naming an
-operator that is not on the hot path is refused before any device work."""
+operator the layer does not provide, or missing a required slot, is refused
+before any device work."""
 import glob
 import json
 import os
@@ -49,6 +50,15 @@ def test_reader_matches_yaml(path):
 
 def test_unknown_operator_is_refused(tmp_path):
     deck = tmp_path / "bad.msp"
-    deck.write_text(open(DECK_FILES[0]).read() + "\nboundary_conditions:\n  - write_paraview\n")
-    r = subprocess.run([RUN, str(deck)], capture_output=True, text=True)
-    assert r.returncode != 0 and "unsupported operator 'write_paraview'" in r.stdout
+    deck.write_text(open(DECK_FILES[0]).read() + "\nboundary_conditions:\n  - init_double_couette\n")
+    r = subprocess.run([RUN, str(deck)], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode != 0 and "unsupported operator 'init_double_couette'" in r.stdout
+
+
+def test_required_analysis_slot_is_checked_before_device_work(tmp_path):
+    # plane_velocity_profile's dimension slot is REQUIRED (plane_velocity_profile.cu:160-161)
+    deck = tmp_path / "bad.msp"
+    deck.write_text(open(DECK_FILES[0]).read() + "\nanalysis:\n  - plane_velocity_profile:\n     dump_file: p.csv\n")
+    r = subprocess.run([RUN, str(deck)], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode != 0 and "the dimension slot is required" in r.stdout
+    assert not (tmp_path / "hippoLBMOutputDir").exists()
