@@ -73,7 +73,30 @@ class Obstacle(ctypes.Structure):
         ("p0", ctypes.c_double * 3),
         ("p1", ctypes.c_double * 3),
         ("radius", ctypes.c_double),
+        ("q", ctypes.c_double * 16),
     ]
+
+
+def make_obstacle(ob):
+    """hlbm_obstacle from a deck obstacle dict (kind ball / wall / quadric)."""
+    o = Obstacle()
+    if ob["kind"] == "ball":
+        o.kind = 0
+        for k in range(3):
+            o.p0[k] = float(ob["center"][k])
+        o.radius = float(ob["radius"])
+    elif ob["kind"] == "wall":
+        o.kind = 1
+        for k in range(3):
+            o.p0[k] = float(ob["bounds"][0][k])
+            o.p1[k] = float(ob["bounds"][1][k])
+    elif ob["kind"] == "quadric":
+        o.kind = 2
+        for k in range(16):
+            o.q[k] = float(ob["q"][k])
+    else:
+        raise ValueError(f"unknown obstacle kind {ob['kind']!r}")
+    return o
 
 
 class TuneInfo(ctypes.Structure):

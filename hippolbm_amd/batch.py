@@ -107,20 +107,7 @@ class Solve:
                 d.cavity_on[side] = 1
                 for k in range(3):
                     d.cavity_u[side][k] = float(u[k])
-        obs = []
-        for ob in self.obstacles:
-            o = _lib.Obstacle()
-            if ob["kind"] == "ball":
-                o.kind = 0
-                for k in range(3):
-                    o.p0[k] = float(ob["center"][k])
-                o.radius = float(ob["radius"])
-            else:
-                o.kind = 1
-                for k in range(3):
-                    o.p0[k] = float(ob["bounds"][0][k])
-                    o.p1[k] = float(ob["bounds"][1][k])
-            obs.append(o)
+        obs = [_lib.make_obstacle(ob) for ob in self.obstacles]
         keep = None
         if obs:
             keep = (_lib.Obstacle * len(obs))(*obs)

@@ -460,27 +460,10 @@ int hlbm_add_obstacles(hlbm_domain* h, const hlbm_obstacle* list, int32_t n) {
   const int nn[3] = {h->nx, h->ny, h->nz};
   std::vector<ObstacleDev> obs;
   for (int k = 0; k < n; k++) {
-    const hlbm_obstacle& o = list[k];
-    if (o.kind != 0 && o.kind != 1) return fail(HLBM_ERR_ARG, "obstacle kind must be 0 (ball) or 1 (wall)");
-    double bmin[3], bmax[3];
-    for (int d = 0; d < 3; d++) {
-      bmin[d] = o.kind == 0 ? o.p0[d] - o.radius : o.p0[d];
-      bmax[d] = o.kind == 0 ? o.p0[d] + o.radius : o.p1[d];
-    }
-    ObstacleDev od{};
-    od.kind = o.kind;
-    bool inside = true;
-    for (int d = 0; d < 3; d++) {
-      // int(b/dx) truncation (update_obstacles.cu:61-62); global node index == our node index
-      const int a = int(bmin[d] / dx), b = int(bmax[d] / dx);
-      if (b < 0 || a > nn[d] - 1) inside = false;
-      od.lo[d] = std::max(a, 0);
-      od.hi[d] = std::min(b, nn[d] - 1);
-      od.p0[d] = o.p0[d];
-      od.p1[d] = o.p1[d];
-    }
-    od.r2 = o.radius * o.radius;
-    if (inside) obs.push_back(od);
+    if (list[k].kind < 0 || list[k].kind > 2)
+      return fail(HLBM_ERR_ARG, "obstacle kind must be 0 (ball), 1 (wall) or 2 (quadric)");
+    ObstacleDev od;
+    if (obstacle_to_dev(list[k], nn, dx, od)) obs.push_back(od);
   }
   if (obs.empty()) return HLBM_OK;
   ObstacleDev* dobs = nullptr;

@@ -386,21 +386,8 @@ static int admit_one(hlbm_batch* b, Solve& s) {
     const int nn[3] = {s.dom.nx, s.dom.ny, s.dom.nz};
     const double dx = s.desc.dx;
     for (const hlbm_obstacle& o : s.obs) {
-      ObstacleDev v{};
-      v.kind = o.kind;
-      bool inside = true;
-      for (int k = 0; k < 3; k++) {
-        const double lo = o.kind == 0 ? o.p0[k] - o.radius : o.p0[k];
-        const double hi = o.kind == 0 ? o.p0[k] + o.radius : o.p1[k];
-        const int a = int(lo / dx), c = int(hi / dx);
-        if (c < 0 || a > nn[k] - 1) inside = false;
-        v.lo[k] = std::max(a, 0);
-        v.hi[k] = std::min(c, nn[k] - 1);
-        v.p0[k] = o.p0[k];
-        v.p1[k] = o.p1[k];
-      }
-      v.r2 = o.radius * o.radius;
-      if (inside) od.push_back(v);
+      ObstacleDev v;
+      if (obstacle_to_dev(o, nn, dx, v)) od.push_back(v);
     }
     if (!od.empty()) {
       // the obstacle list goes through this chunk's pinned staging set

@@ -12,8 +12,9 @@ namespace hlbm {
 // ---------------------------------------------------------------------------
 // update_obstacles (src/obstacle/update_obstacles.cu:32-71): for each obstacle
 // in id order, nodes inside its truncated bounding box (host-computed, already
-// clipped to the grid) are tested with Ball::solid (|p-c|^2 <= r^2) or
-// Wall::solid (strict AABB), obstacle.hpp:13-17,61-64,97.  Position of a node
+// clipped to the grid) are tested with Ball::solid (|p-c|^2 <= r^2),
+// Wall::solid (strict AABB) or Quadric::solid (quadric form <= 0),
+// obstacle.hpp:13-17,61-64,97,136-138.  Position of a node
 // is (local + offset) * dx = node * dx (grid.hpp:355-361).
 // ---------------------------------------------------------------------------
 __global__ void k_rasterize(uint8_t* flags, int nx, int ny, int nz, double dx, const ObstacleDev* obs, int nobs) {
@@ -33,6 +34,11 @@ __global__ void k_rasterize(uint8_t* flags, int nx, int ny, int nz, double dx, c
     if (o.kind == 0) {
       const double rx = px - o.p0[0], ry = py - o.p0[1], rz = pz - o.p0[2];
       if ((rx * rx + ry * ry + rz * rz) <= o.r2) solid = true;
+    } else if (o.kind == 2) {
+      // Quadric::solid: [p,1] Q [p,1]^T <= 0, as p . (Q [p,1]) with the rows of Q first
+      double r[4];
+      for (int i = 0; i < 4; i++) r[i] = o.q[4 * i] * px + o.q[4 * i + 1] * py + o.q[4 * i + 2] * pz + o.q[4 * i + 3];
+      if (px * r[0] + py * r[1] + pz * r[2] + r[3] <= 0.0) solid = true;
     } else {
       if (o.p0[0] < px && px < o.p1[0] && o.p0[1] < py && py < o.p1[1] && o.p0[2] < pz && pz < o.p1[2])
         solid = true;

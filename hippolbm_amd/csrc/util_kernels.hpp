@@ -5,17 +5,49 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/hlbm.h"
+
 namespace hlbm {
 
 struct ObstacleDev {
-  int32_t kind;   // 0 ball, 1 wall
+  int32_t kind;   // 0 ball, 1 wall, 2 quadric
   int32_t lo[3];  // candidate node box (inclusive), already clipped to the grid
   int32_t hi[3];
   int32_t pad;
   double p0[3];   // ball center / wall bmin
   double p1[3];   // wall bmax
   double r2;      // ball radius squared
+  double q[16];   // quadric matrix, row-major
 };
+
+// The device form of one obstacle on an nn[0] x nn[1] x nn[2] node grid of
+// spacing dx; false when its candidate box misses the grid.  Balls and walls
+// take their box from int(bound/dx) (update_obstacles.cu:58-63); a quadric
+// covers everything (Quadric::covered, obstacle.hpp:121-125).
+inline bool obstacle_to_dev(const hlbm_obstacle& o, const int nn[3], double dx, ObstacleDev& od) {
+  od = ObstacleDev{};
+  od.kind = o.kind;
+  bool inside = true;
+  for (int d = 0; d < 3; d++) {
+    if (o.kind == 2) {
+      od.lo[d] = 0;
+      od.hi[d] = nn[d] - 1;
+      continue;
+    }
+    const double bmin = o.kind == 0 ? o.p0[d] - o.radius : o.p0[d];
+    const double bmax = o.kind == 0 ? o.p0[d] + o.radius : o.p1[d];
+    // int(b/dx) truncation; the global node index is the node index
+    const int a = int(bmin / dx), b = int(bmax / dx);
+    if (b < 0 || a > nn[d] - 1) inside = false;
+    od.lo[d] = a > 0 ? a : 0;
+    od.hi[d] = b < nn[d] - 1 ? b : nn[d] - 1;
+    od.p0[d] = o.p0[d];
+    od.p1[d] = o.p1[d];
+  }
+  od.r2 = o.radius * o.radius;
+  for (int k = 0; k < 16; k++) od.q[k] = o.q[k];
+  return inside;
+}
 
 // per-plane statistics of plane_velocity_profile
 struct ProfileStat {

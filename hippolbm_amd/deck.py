@@ -35,6 +35,8 @@ from typing import Any
 
 import yaml
 
+from . import quadrics
+
 _NUM_RE = re.compile(r"^\s*([-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?)\s*[A-Za-z./^0-9-]*\s*$")
 
 HOT_OPERATORS = (
@@ -256,6 +258,14 @@ def from_dict(doc: dict, name: str = "") -> Deck:
             d.obstacles[int(v["id"])] = {"kind": "ball", "center": _vec(v["center"]), "radius": _num(v["radius"])}
         elif k == "register_solid_wall":
             d.obstacles[int(v["id"])] = {"kind": "wall", "bounds": _aabb(v["bounds"])}
+        elif k == "register_quadrics":
+            if "quadrics" not in v or "id" not in v:
+                raise DeckError("register_quadrics: the id and quadrics slots are required")
+            try:
+                q = quadrics.from_slots(v["quadrics"], v.get("transform"))
+            except ValueError as e:
+                raise DeckError(str(e)) from None
+            d.obstacles[int(v["id"])] = {"kind": "quadric", "q": q}
         elif k != "init_obstacles":
             raise DeckError(f"unsupported obstacle operator {k!r}")
     g = doc.get("global") or {}

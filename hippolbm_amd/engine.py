@@ -99,20 +99,7 @@ class Domain:
         check("hlbm_init_obst", lib().hlbm_init_obst(self.h))
 
     def add_obstacles(self, obstacles):
-        arr = []
-        for ob in obstacles:
-            o = Obstacle()
-            if ob["kind"] == "ball":
-                o.kind = 0
-                for k in range(3):
-                    o.p0[k] = float(ob["center"][k])
-                o.radius = float(ob["radius"])
-            else:
-                o.kind = 1
-                for k in range(3):
-                    o.p0[k] = float(ob["bounds"][0][k])
-                    o.p1[k] = float(ob["bounds"][1][k])
-            arr.append(o)
+        arr = [_lib.make_obstacle(ob) for ob in obstacles]
         if not arr:
             return
         buf = (Obstacle * len(arr))(*arr)

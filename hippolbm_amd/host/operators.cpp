@@ -1,10 +1,11 @@
 // operators.cpp -- hippoLBM's hot-path operators over the C ABI; see operators.hpp.
 #include "operators.hpp"
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
-#include <algorithm>
 
 namespace hlbm_host {
 
@@ -201,6 +202,108 @@ struct RegisterWall : OperatorNode {
       o.p0[d] = b[d];
       o.p1[d] = b[3 + d];
     }
+  }
+  void execute(Slots& s) override { s.obstacles[id] = o; }
+};
+
+// ---- register_quadrics (src/obstacle/register_quadrics.cpp:72-84) ----------
+// Q' = M^-T Q M^-1 for the transform list M (applied in order).  The named
+// shapes (sphere, cyl{x,y,z}, cone{x,y,z}, plane{x,y,z}) and the transforms
+// (scale, translate, xrot/yrot/zrot in radians) restate Onika's YAML forms,
+// which are not in the reference tree.  M^-1 is composed from each
+// transform's inverse in reverse order; products sum k = 0..3 in order, as
+// hippolbm_amd/quadrics.py does.
+using Mat4 = std::array<std::array<double, 4>, 4>;
+
+Mat4 mat_identity() {
+  Mat4 m{};
+  for (int i = 0; i < 4; i++) m[i][i] = 1.0;
+  return m;
+}
+
+Mat4 mat_mul(const Mat4& a, const Mat4& b) {
+  Mat4 o{};
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) {
+      double acc = 0.0;
+      for (int k = 0; k < 4; k++) acc += a[i][k] * b[k][j];
+      o[i][j] = acc;
+    }
+  return o;
+}
+
+Mat4 named_quadric(const std::string& n) {
+  Mat4 q{};
+  const std::map<std::string, std::array<double, 4>> diag = {
+      {"sphere", {1, 1, 1, -1}}, {"cylx", {0, 1, 1, -1}},  {"cyly", {1, 0, 1, -1}}, {"cylz", {1, 1, 0, -1}},
+      {"conex", {-1, 1, 1, 0}},  {"coney", {1, -1, 1, 0}}, {"conez", {1, 1, -1, 0}}};
+  auto it = diag.find(n);
+  if (it != diag.end()) {
+    for (int i = 0; i < 4; i++) q[i][i] = it->second[i];
+    return q;
+  }
+  const int a = n == "planex" ? 0 : n == "planey" ? 1 : n == "planez" ? 2 : -1;
+  if (a < 0) throw OperatorError("register_quadrics: unknown quadric '" + n + "'");
+  q[a][3] = q[3][a] = 0.5;
+  return q;
+}
+
+Mat4 transform_inverse(const std::string& op, const msp::Node& v) {
+  Mat4 m = mat_identity();
+  if (op == "scale") {
+    const std::vector<double> s = vec3(v);
+    for (int d = 0; d < 3; d++) m[d][d] = 1.0 / s[d];
+  } else if (op == "translate") {
+    const std::vector<double> t = vec3(v);
+    for (int d = 0; d < 3; d++) m[d][3] = -t[d];
+  } else if (op == "xrot" || op == "yrot" || op == "zrot") {
+    const int axis = op[0] - 'x';
+    const double c = std::cos(v.number()), s = -std::sin(v.number());
+    const int i = axis == 0 ? 1 : axis == 1 ? 2 : 0, j = axis == 0 ? 2 : axis == 1 ? 0 : 1;
+    m[i][i] = c;
+    m[i][j] = -s;
+    m[j][i] = s;
+    m[j][j] = c;
+  } else {
+    throw OperatorError("register_quadrics: unknown transform '" + op + "'");
+  }
+  return m;
+}
+
+struct RegisterQuadric : OperatorNode {
+  long id;
+  hlbm_obstacle o{};
+  explicit RegisterQuadric(const msp::Node& a) {
+    if (!a.find("id") || !a.find("quadrics")) throw OperatorError("register_quadrics: the id and quadrics slots are required");
+    id = a.at("id").integer();
+    const msp::Node& qn = a.at("quadrics");
+    Mat4 q{};
+    if (qn.is_scalar()) {
+      q = named_quadric(qn.text);
+    } else {
+      if (!qn.is_seq() || qn.items.size() != 4) throw OperatorError("register_quadrics: quadrics must be a name or a 4x4 matrix");
+      for (int i = 0; i < 4; i++) {
+        const std::vector<double> row = qn.items[i].numbers();
+        if (row.size() != 4) throw OperatorError("register_quadrics: quadrics must be a name or a 4x4 matrix");
+        for (int j = 0; j < 4; j++) q[i][j] = row[j];
+      }
+    }
+    if (const msp::Node* t = a.find("transform")) {
+      if (!t->is_seq()) throw OperatorError("register_quadrics: transform must be a list");
+      Mat4 minv = mat_identity();
+      for (const auto& step : t->items) {
+        if (!step.is_map() || step.fields.size() != 1)
+          throw OperatorError("register_quadrics: each transform is {scale|translate|xrot|yrot|zrot: value}");
+        minv = mat_mul(minv, transform_inverse(step.fields[0].first, step.fields[0].second));
+      }
+      Mat4 mt{};
+      for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) mt[i][j] = minv[j][i];
+      q = mat_mul(mat_mul(mt, q), minv);
+    }
+    o.kind = 2;
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) o.q[4 * i + j] = q[i][j];
   }
   void execute(Slots& s) override { s.obstacles[id] = o; }
 };
@@ -402,6 +505,7 @@ struct Registrar {
     f->register_factory("init_obstacles", plain<InitObstacles>());
     f->register_factory("register_solid_ball", with_args<RegisterBall>());
     f->register_factory("register_solid_wall", with_args<RegisterWall>());
+    f->register_factory("register_quadrics", with_args<RegisterQuadric>());
     f->register_factory("update_obstacles", plain<UpdateObstacles>());
     f->register_factory("macro_variables", plain<MacroVariables>());
     f->register_factory("bgk", [](const msp::Node&) { return std::unique_ptr<OperatorNode>(new Collision(HLBM_BGK)); });

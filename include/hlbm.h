@@ -108,11 +108,14 @@ typedef struct {
 } hlbm_bc;
 
 typedef struct {
-  int32_t kind; /* 0 = ball (register_solid_ball), 1 = wall (register_solid_wall)              */
+  int32_t kind; /* 0 = ball (register_solid_ball), 1 = wall (register_solid_wall),
+                   2 = quadric (register_quadrics)                                             */
   int32_t pad;
   double p0[3]; /* ball: center; wall: bmin                                                    */
   double p1[3]; /* wall: bmax                                                                  */
   double radius;
+  double q[16]; /* quadric: 4x4 matrix Q, row-major, transform already applied
+                   (register_quadrics.cpp:72-84); solid where [p,1] Q [p,1]^T <= 0            */
 } hlbm_obstacle;
 
 /* ---- library ---------------------------------------------------------- */
@@ -146,8 +149,9 @@ int hlbm_set_collision(hlbm_domain* h, int32_t collision, const double* mrt_rate
 /* ---- initial state ----------------------------------------------------- */
 /* init_obst: every node fluid (src/grid/init_obst.cu:54-60). */
 int hlbm_init_obst(hlbm_domain* h);
-/* update_obstacles for a list of balls/walls in obstacle-id order, rasterized
- * on the device with the reference's rules (src/obstacle/update_obstacles.cu:32-71). */
+/* update_obstacles for a list of balls/walls/quadrics in obstacle-id order,
+ * rasterized on the device with the reference's rules
+ * (src/obstacle/update_obstacles.cu:32-71, obstacle.hpp:13-17,61-64,97,136-138). */
 int hlbm_add_obstacles(hlbm_domain* h, const hlbm_obstacle* list, int32_t n);
 /* Obstacle field as int32 -1 (fluid) / -2 (wall), node or reference layout. */
 int hlbm_set_flags(hlbm_domain* h, const int32_t* obst, int32_t layout);

@@ -356,6 +356,40 @@ void orc_update_obstacle(void* h, int kind, const double* p0, const double* p1, 
       }
 }
 
+// Quadric obstacle (register_quadrics.cpp:72-84, obstacle.hpp:107-138): the
+// candidate box is Quadric::covered() = [-1e6, 1e6]^3 truncated by int(b/dx)
+// (clamped to the int range) and restricted to the Extend box; solid where
+// [p,1] Q [p,1]^T <= 0.  onika::math::quadric_eval is not in the reference
+// tree: the form is evaluated as p . (Q [p,1]) with the rows of Q first, left
+// to right -- parity unpinned at the rounding level of that order.
+void orc_update_quadric(void* h, const double* q) {
+  Grid* g = (Grid*)h;
+  const double dx = g->dx;
+  auto trunc = [](double v) {
+    if (v > 2147483647.0) return 2147483647;
+    if (v < -2147483647.0) return -2147483647;
+    return int(v);
+  };
+  int gmin[3], gmax[3];
+  for (int d = 0; d < 3; d++) {
+    gmin[d] = trunc(-1e6 / dx);
+    gmax[d] = trunc(1e6 / dx);
+  }
+  int lo[3], hi[3];
+  bool ok;
+  restrict_to_extend(g, gmin, gmax, lo, hi, &ok);
+  if (!ok) return;
+  for (int k = lo[2]; k <= hi[2]; k++)
+    for (int j = lo[1]; j <= hi[1]; j++)
+      for (int i = lo[0]; i <= hi[0]; i++) {
+        const double x[3] = {(double(i) + double(-GHOST)) * dx, (double(j) + double(-GHOST)) * dx,
+                             (double(k) + double(-GHOST)) * dx};
+        double r[4];
+        for (int a = 0; a < 4; a++) r[a] = q[4 * a] * x[0] + q[4 * a + 1] * x[1] + q[4 * a + 2] * x[2] + q[4 * a + 3];
+        if (x[0] * r[0] + x[1] * r[1] + x[2] * r[2] + r[3] <= 0.0) g->obst[g->idx(i, j, k)] = WALL;
+      }
+}
+
 // ---------------------------------------------------------------------------
 // macro_variables, src/collision/include/hippoLBM/collision/macro_variables.hpp:48-79
 // (All points)

@@ -67,6 +67,7 @@ def lib():
             "orc_set_node_populations": (None, [P, DP]),
             "orc_get_node_populations": (None, [P, DP]),
             "orc_update_obstacle": (None, [P, I, DP, DP, D]),
+            "orc_update_quadric": (None, [P, DP]),
             "orc_macro_variables": (None, [P]),
             "orc_bgk": (None, [P, D, D, D, D]),
             "orc_mrt": (None, [P, D, D, D, D, DP]),
@@ -169,6 +170,10 @@ class Oracle:
         b = np.asarray(p1, dtype=np.float64)
         self.L.orc_update_obstacle(self.h, {"ball": 0, "wall": 1}[kind], _dp(a), _dp(b), float(r))
 
+    def update_quadric(self, q):
+        a = np.ascontiguousarray(q, dtype=np.float64).reshape(16)
+        self.L.orc_update_quadric(self.h, _dp(a))
+
     def macro_variables(self):
         self.L.orc_macro_variables(self.h)
 
@@ -241,6 +246,8 @@ class OracleRun:
         for ob in deck.obstacles_in_id_order():
             if ob["kind"] == "ball":
                 o.update_obstacle("ball", ob["center"], r=ob["radius"])
+            elif ob["kind"] == "quadric":
+                o.update_quadric(ob["q"])
             else:
                 o.update_obstacle("wall", ob["bounds"][0], ob["bounds"][1])
         if init_f is not None:

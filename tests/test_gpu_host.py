@@ -73,3 +73,19 @@ def test_failing_checker_exits_with_failure(tmp_path):
     r = run([str(deck), "--quiet"], tmp_path)
     assert r.returncode != 0
     assert "Regression test checker [macro quantities]" in r.stdout and "Sum of densities" in r.stdout
+
+
+def test_quadric_deck_runs_through_the_operator_layer(tmp_path):
+    from test_gpu_parity import QUADRIC_DECK
+    deck = tmp_path / "quadrics.msp"
+    deck.write_text(QUADRIC_DECK)
+    r = run([str(deck), "--quiet", "--no-output"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    step, s, vmin, vmax = final_state(r.stdout)
+    d = dk.loads_msp(QUADRIC_DECK, "quadrics")
+    ref = orc.OracleRun(d)
+    ref.step(d.end_iteration)
+    o = ref.final_state()
+    assert step == d.end_iteration
+    assert abs(s - o[0]) <= 1e-12 * abs(o[0])
+    assert vmin == o[1] and vmax == o[2]

@@ -269,6 +269,70 @@ def test_wide_address_variant_bitwise(pattern, monkeypatch):
     sim.dom.close()
 
 
+# quadric obstacles (register_quadrics): a cylinder across the closed z axis,
+# an ellipsoid and a narrow double cone, all clear of the periodic x/y faces
+QUADRIC_DECK = """
+do_domain:
+  - domain:
+     bounds: [[0,0,0],[0.5,0.2,0.3]]
+     cell_dims: [50, 20, 30]
+     periodic: [true, true, false]
+set_lbm_parameters:
+  - lbm_parameters:
+     Fext: [1.3, 0, 0]
+     nuth: 1e-3
+     tau: 0.65
+set_obstacles:
+  - register_quadrics:
+     id: 0
+     quadrics: cylz
+     transform:
+        - scale: [ 0.03, 0.03, 1 ]
+        - translate: [ 0.12, 0.1, 0 ]
+  - register_quadrics:
+     id: 1
+     quadrics: sphere
+     transform:
+        - scale: [ 0.04, 0.05, 0.04 ]
+        - translate: [ 0.27, 0.1, 0.15 ]
+  - register_quadrics:
+     id: 2
+     quadrics: conez
+     transform:
+        - scale: [ 0.01, 0.01, 0.05 ]
+        - zrot: 0.3
+        - translate: [ 0.4, 0.1, 0.2 ]
+boundary_conditions:
+  - neumann:
+     U: [0,0,0]
+     regions: [plan_xy_l]
+  - neumann:
+     U: [0,0,0]
+     regions: [plan_xy_0]
+pre_stream_bcs:
+  - wall_bounce_back
+global:
+   simulation_end_iteration: 30
+   simulation_print_log_freq: -1
+"""
+
+
+@pytest.mark.parametrize("pattern", ["ab", "aa"])
+def test_quadric_obstacles_bitwise(pattern):
+    d = dk.loads_msp(QUADRIC_DECK, "quadrics")
+    say("quadrics", pattern)
+    ref = orc.OracleRun(d)
+    sim = Simulation(d, storage="f64", pattern=pattern)
+    flags = sim.dom.get_flags()
+    assert np.array_equal(flags, ref.o.real(ref.o.obst()))
+    assert (flags == -2).sum() > 1000
+    ref.step(30)
+    sim.run(30)
+    nbad, maxdiff, _ = compare(sim, ref, fluid_only=True)
+    assert nbad == 0, (nbad, maxdiff)
+    sim.dom.close()
+
+
 def test_outflow_after_an_operator_on_its_interior_layer_is_rejected():
     d = _channel_deck(20, 10, 10, 0, 1, "bgk")
     d.boundary_conditions.insert(0, ("lid_driven_cavity", {"U": (0.01, 0.0, 0.0), "regions": ["plan_xy_l"]}))

@@ -21,6 +21,7 @@ import pytest
 
 import oracle as orc
 from hippolbm_amd import deck as dk
+from hippolbm_amd import quadrics
 
 from conftest import DECKS
 
@@ -100,6 +101,78 @@ def test_ball_rasterization_brute_force():
     # brute force differs only where |p-c|^2 is within rounding of r^2
     assert (got != solid).sum() <= 2
     assert got.sum() > 50
+
+
+def _quadric_grid(n, dx):
+    z, y, x = np.meshgrid(np.arange(n[2]), np.arange(n[1]), np.arange(n[0]), indexing="ij")
+    return x * dx, y * dx, z * dx
+
+
+def test_transformed_quadric_closed_forms():
+    # scale s then translate t maps the unit sphere to sum(((p-t)/s)^2) - 1
+    rng = np.random.default_rng(3)
+    s, t = (0.05, 0.08, 0.05), (0.35, 0.1, 0.15)
+    q = np.array(quadrics.from_slots("sphere", [{"scale": list(s)}, {"translate": list(t)}])).reshape(4, 4)
+    for p in rng.uniform(-1, 1, (50, 3)):
+        exact = sum(((p[d] - t[d]) / s[d]) ** 2 for d in range(3)) - 1.0
+        assert abs(quadrics.evaluate(q, p) - exact) <= 1e-9 * max(1.0, abs(exact))
+    # a cylinder along z turned by xrot(pi/2) is a cylinder along y
+    qa = np.array(quadrics.from_slots("cylz", [{"xrot": np.pi / 2}])).reshape(4, 4)
+    qb = np.array(quadrics.from_slots("cyly")).reshape(4, 4)
+    for p in rng.uniform(-2, 2, (50, 3)):
+        assert abs(quadrics.evaluate(qa, p) - quadrics.evaluate(qb, p)) <= 1e-12
+    # a cone along z: solid where x^2 + y^2 <= z^2; a plane: solid where x <= 0
+    qc = quadrics.named("conez")
+    assert quadrics.evaluate(qc, (0.3, 0.4, 0.5)) == 0.0 and quadrics.evaluate(qc, (0.3, 0.4, 0.6)) < 0
+    assert quadrics.evaluate(quadrics.named("planex"), (-0.1, 5, 5)) < 0 < quadrics.evaluate(quadrics.named("planex"), (0.1, 0, 0))
+    with pytest.raises(ValueError):
+        quadrics.from_slots("torus")
+
+
+def test_quadric_rasterization_brute_force():
+    # a scaled + translated sphere rasterizes like the ball it describes, and a
+    # cylinder along y like its disc in x-z, except at rounding ties
+    n, dx = (41, 21, 31), 0.01
+    o = orc.Oracle(n, (True, True, False), dx)
+    o.init_obst()
+    c, r = (0.1513, 0.1027, 0.1191), 0.0537  # no node on the surface
+    o.update_quadric(quadrics.from_slots("sphere", [{"scale": [r, r, r]}, {"translate": list(c)}]))
+    x, y, z = _quadric_grid(n, dx)
+    ball = (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2 <= r * r
+    got = o.real(o.obst()) == -2
+    assert ball.sum() > 300 and (got != ball).sum() <= 2
+    o.init_obst()
+    o.update_quadric(quadrics.from_slots("cyly", [{"scale": [0.0537, 1, 0.0537]}, {"translate": [0.2013, 0.1, 0.1071]}]))
+    disc = (x - 0.2013) ** 2 + (z - 0.1071) ** 2 <= 0.0537 ** 2
+    got = o.real(o.obst()) == -2
+    assert disc.sum() > 1000 and (got != disc).sum() <= 2 * n[1]
+    # the quadric covers the periodic ghost layers too (Extend box)
+    assert (o.obst()[2:-2, :, 0:2] == -2).sum() == 0  # disc away from x = 0
+    o.init_obst()
+    o.update_quadric(quadrics.from_slots("planex", [{"translate": [0.005, 0, 0]}]))
+    assert (o.obst()[2:-2, 2:-2, :] == -2).sum(axis=2).max() == 3  # nodes x = -2, -1, 0 (ghosts included)
+
+
+def test_quadric_deck_parses_to_obstacles():
+    txt = open(os.path.join(DECKS, "regression_lbm_cavite.msp")).read()
+    txt += """
+set_obstacles:
+  - register_quadrics:
+     id: 1
+     quadrics: conez
+     transform:
+        - scale: [ 0.05 , 0.05, 0.05 ]
+        - translate: [ 0.55 , 0.1, 0.25 ]
+  - register_solid_ball:
+     id: 0
+     center: [1, 1, 1]
+     radius: 0.5
+"""
+    d = dk.loads_msp(txt, "q")
+    obs = d.obstacles_in_id_order()
+    assert [o["kind"] for o in obs] == ["ball", "quadric"] and len(obs[1]["q"]) == 16
+    with pytest.raises(dk.DeckError):
+        dk.loads_msp(txt.replace("conez", "torus"), "bad")
 
 
 # ---------------------------------------------------------------------------
