@@ -21,8 +21,8 @@ STORAGE_BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2}
 COLLISION = {"bgk": 0, "mrt": 1}
 LAYOUT_NODES, LAYOUT_REF = 0, 1
 BC_KIND = {"pre_bounce_back": 1, "post_bounce_back": 2, "wall_bounce_back": 3, "cavity_z_0": 4, "cavity_z_l": 5,
-           "lid_driven_cavity": 6, "neumann": 7, "pressure_inlet": 8, "outflow": 9}
-STAGE = {"pre_stream_bcs": 0, "post_stream_bcs": 1, "boundary_conditions": 2}
+           "lid_driven_cavity": 6, "neumann": 7, "pressure_inlet": 8, "outflow": 9, "set_distribution": 10}
+STAGE = {"start_iteration": -1, "pre_stream_bcs": 0, "post_stream_bcs": 1, "boundary_conditions": 2}
 PLANE = {"plan_xy_0": 0, "plan_xy_l": 1, "plan_xz_0": 2, "plan_xz_l": 3, "plan_yz_0": 4, "plan_yz_l": 5}
 STEP_WRITE_MACRO = 1
 
@@ -63,6 +63,7 @@ class BC(ctypes.Structure):
         ("plane", ctypes.c_int32),
         ("pad", ctypes.c_int32),
         ("U", ctypes.c_double * 3),
+        ("bounds", ctypes.c_double * 6),
     ]
 
 

@@ -304,12 +304,32 @@ int hlbm_set_bcs(hlbm_domain* h, const hlbm_bc* list, int32_t n) {
   bool pre_bb = false, post_bb = false;
   bool cav_after_pre[2] = {false, false};
   double cav_u[2][3] = {{0, 0, 0}, {0, 0, 0}};
-  int last_stage = 0;
+  int last_stage = HLBM_STAGE_START;
   for (int k = 0; k < n; k++) {
     const hlbm_bc& b = list[k];
-    if (b.stage < last_stage) return fail(HLBM_ERR_ARG, "bc list must be ordered pre_stream, post_stream, boundary");
+    if (b.stage < last_stage)
+      return fail(HLBM_ERR_ARG, "bc list must be ordered start_iteration, pre_stream, post_stream, boundary");
     last_stage = b.stage;
-    if (b.stage == HLBM_STAGE_PRE_STREAM) {
+    if (b.stage == HLBM_STAGE_START) {
+      if (b.kind != HLBM_BC_SET_DISTRIBUTION)
+        return fail(HLBM_ERR_UNSUPPORTED, "only set_distribution may run in start_iteration");
+      if (P.nset >= MAX_SET) return fail(HLBM_ERR_UNSUPPORTED, "at most 4 set_distribution operators per step");
+      int* B = P.set_box[P.nset];
+      const int nn[3] = {h->nx, h->ny, h->nz};
+      bool inside = true;
+      for (int d = 0; d < 3; d++) {
+        B[d] = 0;
+        B[3 + d] = nn[d] - 1;
+        if (b.plane == 1) {  // bounded: int(b/dx) box restricted to the grid (set_distribution.cu:84-95)
+          if (!(h->d.dx > 0.0)) return fail(HLBM_ERR_ARG, "desc.dx must be > 0 for a bounded set_distribution");
+          const int a = int(b.bounds[d] / h->d.dx), c = int(b.bounds[3 + d] / h->d.dx);
+          if (c < 0 || a > nn[d] - 1) inside = false;
+          B[d] = std::max(a, 0);
+          B[3 + d] = std::min(c, nn[d] - 1);
+        }
+      }
+      if (inside) P.set_val[P.nset++] = b.U[0];
+    } else if (b.stage == HLBM_STAGE_PRE_STREAM) {
       switch (b.kind) {
         case HLBM_BC_PRE_BOUNCE_BACK:
           pre_bb = true;

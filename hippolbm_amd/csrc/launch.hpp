@@ -12,10 +12,12 @@ namespace hlbm {
 // The bulk kernel's fast variant addresses a plane with 32-bit byte offsets
 // and reads obstacle bounce-back from the precomputed per-node mask.  The
 // general variant (64-bit addresses, neighbour flags read directly) runs when
-// some offset, (N-1)*element size, needs more than 32 bits, or when wall
-// bounce-back is active and no mask was supplied.
+// some offset, (N-1)*element size, needs more than 32 bits, when wall
+// bounce-back is active and no mask was supplied, or when start_iteration
+// boxes are set (the 32-bit bulk path leaves them out).
 inline bool wide_offsets(const StepParams& p, size_t esize) {
   if (p.force_wide) return true;
+  if (p.nset) return true;  // start_iteration boxes are applied by finish_node, which the wide variant uses
   if (p.any_wall && p.wall_bb && !p.bbm) return true;
   return (uint64_t)(p.N - 1u) * esize > 0xFFFFFFFFull;
 }

@@ -62,6 +62,8 @@ struct ProgEntry {
   double U[3];
 };
 
+constexpr int MAX_SET = 4;
+
 struct StepParams {
   int32_t nx, ny, nz;
   int32_t per[3];
@@ -101,6 +103,9 @@ struct StepParams {
   double ef[Q];         // c_i . F, evaluated in the reference's order on the host
   double mrt[10];       // s1, s2, s4, s9, s10, s13, s16, weps, wepsj, wxx (mrt_custom)
   ProgEntry prog[MAX_PROG];
+  int32_t nset;                // start_iteration set_distribution boxes
+  int32_t set_box[MAX_SET][6]; // node box, inclusive: x0 y0 z0 x1 y1 z1
+  double set_val[MAX_SET];     // f_i = value * w_i
 };
 
 // ---------------------------------------------------------------------------
@@ -901,9 +906,30 @@ __device__ __forceinline__ void store_node(const StepParams& p, int x, int y, in
   }
 }
 
+// start_iteration set_distribution (set_distribution.hpp:38-41): before the
+// macro variables of a step, f_i = value * w_i on the nodes of each box, in
+// list order; the shifted storage holds f_i - w_i.
+template <class S, int MODE>
+__device__ __forceinline__ void apply_sets(const StepParams& p, int x, int y, int z, typename StoreOps<S>::C* f) {
+  using C = typename StoreOps<S>::C;
+  if constexpr (mode_collides(MODE)) {
+    for (int b = 0; b < p.nset; b++) {
+      const int* B = p.set_box[b];
+      const bool in = x >= B[0] && x <= B[3] && y >= B[1] && y <= B[4] && z >= B[2] && z <= B[5];
+      const double v = p.set_val[b];
+#pragma unroll
+      for (int i = 0; i < Q; i++) {
+        const C nv = StoreOps<S>::shifted ? C(v * WD(i) - WD(i)) : C(v * WD(i));
+        f[i] = in ? nv : f[i];
+      }
+    }
+  }
+}
+
 template <class S, int COLL, int MODE>
 __device__ __forceinline__ void finish_node(const StepParams& p, int x, int y, int z, uint32_t idx, bool wall,
                                             typename StoreOps<S>::C* f) {
+  if (p.nset) apply_sets<S, MODE>(p, x, y, z, f);
   collide_node<S, COLL, MODE>(p, idx, wall, f);
   store_node<S, MODE>(p, x, y, z, idx, wall, f);
 }
@@ -1369,6 +1395,7 @@ __device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, c
   p.flags = bp.flags + d.flag_off;
   p.m0 = nullptr;
   p.m1 = nullptr;
+  p.nset = 0;
   p.write_macro = 0;
   p.wall_bb = d.wall_bb;
   p.plane_bb = 1;

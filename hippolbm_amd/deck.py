@@ -112,6 +112,7 @@ class Deck:
     post_stream_bcs: list = dataclasses.field(default_factory=list)
     boundary_conditions: list = dataclasses.field(default_factory=list)
     set_distributions: list = dataclasses.field(default_factory=list)
+    start_iteration: list = dataclasses.field(default_factory=list)  # set_distribution entries
     obstacles: dict = dataclasses.field(default_factory=dict)
     end_iteration: int = 100
     checker: dict | None = None
@@ -253,6 +254,15 @@ def from_dict(doc: dict, name: str = "") -> Deck:
             if "bounds" in v:
                 sd["bounds"] = _aabb(v["bounds"])
             d.set_distributions.append(sd)
+    # start_iteration (config_compute_loop.msp:27): operators at the start of every step
+    for k, v in _items(doc.get("start_iteration")):
+        if k == "set_distribution":
+            sd = {"value": _num((v or {}).get("value", 1.0))}
+            if v and "bounds" in v:
+                sd["bounds"] = _aabb(v["bounds"])
+            d.start_iteration.append(sd)
+        elif k != "nop":
+            raise DeckError(f"unsupported start_iteration operator {k!r}")
     for k, v in _items(doc.get("set_obstacles")):
         if k == "register_solid_ball":
             d.obstacles[int(v["id"])] = {"kind": "ball", "center": _vec(v["center"]), "radius": _num(v["radius"])}

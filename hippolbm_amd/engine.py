@@ -77,6 +77,18 @@ class Domain:
         for stage, name, args in entries:
             if name in ("nop", "update_ghost"):
                 continue
+            if name == "set_distribution":  # start_iteration: U[0] = value, plane = bounded
+                b = BC()
+                b.kind = _lib.BC_KIND[name]
+                b.stage = _lib.STAGE[stage]
+                b.U[0] = float(args.get("value", 1.0))
+                if args.get("bounds") is not None:
+                    b.plane = 1
+                    for k in range(3):
+                        b.bounds[k] = float(args["bounds"][0][k])
+                        b.bounds[3 + k] = float(args["bounds"][1][k])
+                arr.append(b)
+                continue
             regions = args.get("regions") or [None]
             for r in regions:
                 b = BC()
@@ -219,7 +231,7 @@ def bc_entries(deck):
     """The three operator lists of a deck as (stage, name, args) with LBM-unit velocities
     (lid_driven_cavity.cu:75 and neuman.cu:74 divide by celerity; cavity.cu:70 does not)."""
     c = deck.params["celerity"]
-    out = []
+    out = [("start_iteration", "set_distribution", dict(sd)) for sd in deck.start_iteration]
     for stage in ("pre_stream_bcs", "post_stream_bcs", "boundary_conditions"):
         for name, args in getattr(deck, stage):
             a = dict(args)

@@ -167,6 +167,18 @@ struct SetDistribution : OperatorNode {
     }
   }
   void execute(Slots& s) override {
+    if (s.in_loop && s.stage == HLBM_STAGE_START) {
+      // start_iteration: the step kernel applies it to every step of the plan
+      hlbm_bc e{};
+      e.kind = HLBM_BC_SET_DISTRIBUTION;
+      e.stage = HLBM_STAGE_START;
+      e.plane = has_bounds ? 1 : 0;
+      e.U[0] = value;
+      if (has_bounds)
+        for (int k = 0; k < 6; k++) e.bounds[k] = b[k];
+      s.recording.push_back(e);
+      return;
+    }
     check(hlbm_set_distribution(s.dom, value, has_bounds ? b : nullptr), "set_distribution");
     if (!has_bounds) s.set_dist_value = value;
   }
@@ -614,6 +626,7 @@ Simulation::Simulation(const msp::Node& deck) {
   // f stays at rho = 1), so it is not read.
   set_distributions = make_list(deck.find("set_distributions"));
   set_obstacles = make_list(deck.find("set_obstacles"));
+  start_iteration = make_list(deck.find("start_iteration"));
   checker_op = make_list(deck.find("checker_op"));
   analysis = make_list(deck.find("analysis"));
   if (const msp::Node* c = deck.find("checker")) {
@@ -677,6 +690,8 @@ void Simulation::compute_loop() {
   Slots& s = slots;
   s.in_loop = true;
   while (s.timestep <= s.end_iteration) {
+    s.stage = HLBM_STAGE_START;
+    run(start_iteration, s);
     MacroVariables().execute(s);
     run(collision, s);
     s.stage = HLBM_STAGE_PRE_STREAM;
@@ -690,7 +705,7 @@ void Simulation::compute_loop() {
     if (!s.plan_set || s.recording.size() != s.plan.size() ||
         !std::equal(s.recording.begin(), s.recording.end(), s.plan.begin(), [](const hlbm_bc& a, const hlbm_bc& b) {
           return a.kind == b.kind && a.stage == b.stage && a.plane == b.plane && a.U[0] == b.U[0] &&
-                 a.U[1] == b.U[1] && a.U[2] == b.U[2];
+                 a.U[1] == b.U[1] && a.U[2] == b.U[2] && std::equal(a.bounds, a.bounds + 6, b.bounds);
         })) {
       s.flush();
       check(hlbm_set_bcs(s.dom, s.recording.data(), (int32_t)s.recording.size()), "boundary conditions");

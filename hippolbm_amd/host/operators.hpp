@@ -133,7 +133,8 @@ std::vector<std::unique_ptr<OperatorNode>> make_list(const msp::Node* list);
 struct Simulation {
   Slots slots;
   std::vector<std::unique_ptr<OperatorNode>> do_domain, set_lbm_parameters, collision, pre_stream_bcs,
-      post_stream_bcs, boundary_conditions, set_distributions, set_obstacles, checker_op, analysis, checker_list;
+      post_stream_bcs, boundary_conditions, set_distributions, set_obstacles, checker_op, analysis, checker_list,
+      start_iteration;
   bool checker_override = false;  // the deck redefines `checker`
   std::unique_ptr<OperatorNode> paraview;  // write_paraview with the default slots
 

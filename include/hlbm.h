@@ -65,10 +65,18 @@ enum hlbm_bc_kind {
    *   outflow: zero-gradient outlet, the populations entering from outside
    *     copy their post-stream values at the first interior node. */
   HLBM_BC_PRESSURE_INLET = 8,    /* "pressure_inlet"                                        */
-  HLBM_BC_OUTFLOW = 9            /* "outflow"                                               */
+  HLBM_BC_OUTFLOW = 9,           /* "outflow"                                               */
+  /* set_distribution in start_iteration (src/grid/set_distribution.cu:69-109,
+   * example/ball.msp:23-26): at the start of every step, f_i = U[0] * w_i on the
+   * nodes of the box `bounds` (plane = 1) or on every node (plane = 0); stage
+   * HLBM_STAGE_START only, at most 4 entries, applied in list order. */
+  HLBM_BC_SET_DISTRIBUTION = 10  /* "set_distribution"                                      */
 };
-/* operator lists of the compute loop (data/config/config_compute_loop.msp:28-36) */
-enum hlbm_bc_stage { HLBM_STAGE_PRE_STREAM = 0, HLBM_STAGE_POST_STREAM = 1, HLBM_STAGE_BOUNDARY = 2 };
+/* operator lists of the compute loop (data/config/config_compute_loop.msp:23-38):
+ * start_iteration runs before macro_variables and the collision */
+enum hlbm_bc_stage {
+  HLBM_STAGE_START = -1, HLBM_STAGE_PRE_STREAM = 0, HLBM_STAGE_POST_STREAM = 1, HLBM_STAGE_BOUNDARY = 2
+};
 /* plane traversals (src/grid/include/hippoLBM/grid/grid_region.hpp:75-77) */
 enum hlbm_plane {
   HLBM_PLAN_XY_0 = 0, HLBM_PLAN_XY_L = 1, HLBM_PLAN_XZ_0 = 2,
@@ -105,6 +113,7 @@ typedef struct {
   int32_t pad;
   double U[3];   /* velocity in LBM units for lid/neumann (already divided by celerity);
                     raw operator value for cavity_z_* (hippoLBM applies no conversion there) */
+  double bounds[6]; /* set_distribution: {xmin,ymin,zmin,xmax,ymax,zmax} (physical units)    */
 } hlbm_bc;
 
 typedef struct {

@@ -260,6 +260,8 @@ class OracleRun:
     def step(self, n=1):
         o, d, p = self.o, self.deck, self.p
         for _ in range(n):
+            for sd in d.start_iteration:  # config_compute_loop.msp:27
+                o.set_distribution(sd.get("value", 1.0), sd.get("bounds"), False)
             o.macro_variables()
             if d.collision == "mrt":
                 o.mrt(p["tau"], p["Fext"], d.mrt_rates)

@@ -42,8 +42,9 @@ def test_struct_layout_matches_header(tmp_path):
         "int main(void){\n"
         'printf("%zu %zu %zu %zu %zu\\n", sizeof(hlbm_domain_desc), offsetof(hlbm_domain_desc, tau),'
         " offsetof(hlbm_domain_desc, dx), offsetof(hlbm_domain_desc, mrt_rates), offsetof(hlbm_domain_desc, fext));\n"
-        'printf("%zu %zu\\n", sizeof(hlbm_bc), offsetof(hlbm_bc, U));\n'
-        'printf("%zu %zu %zu\\n", sizeof(hlbm_obstacle), offsetof(hlbm_obstacle, p1), offsetof(hlbm_obstacle, radius));\n'
+        'printf("%zu %zu %zu\\n", sizeof(hlbm_bc), offsetof(hlbm_bc, U), offsetof(hlbm_bc, bounds));\n'
+        'printf("%zu %zu %zu %zu\\n", sizeof(hlbm_obstacle), offsetof(hlbm_obstacle, p1), offsetof(hlbm_obstacle, radius),'
+        ' offsetof(hlbm_obstacle, q));\n'
         'printf("%zu %zu %zu %zu\\n", sizeof(hlbm_solve_desc), offsetof(hlbm_solve_desc, tau),'
         ' offsetof(hlbm_solve_desc, cavity_u), offsetof(hlbm_solve_desc, obstacles));\n'
         'printf("%zu %zu %zu\\n", sizeof(hlbm_batch_desc), sizeof(hlbm_batch_info), offsetof(hlbm_batch_info, gpu_ms));\n'
@@ -56,9 +57,10 @@ def test_struct_layout_matches_header(tmp_path):
     D = _lib.DomainDesc
     assert d == [ctypes.sizeof(D), D.tau.offset, D.dx.offset, D.mrt_rates.offset, D.fext.offset]
     b = [int(v) for v in lines[1].split()]
-    assert b == [ctypes.sizeof(_lib.BC), _lib.BC.U.offset]
+    assert b == [ctypes.sizeof(_lib.BC), _lib.BC.U.offset, _lib.BC.bounds.offset]
     o = [int(v) for v in lines[2].split()]
-    assert o == [ctypes.sizeof(_lib.Obstacle), _lib.Obstacle.p1.offset, _lib.Obstacle.radius.offset]
+    assert o == [ctypes.sizeof(_lib.Obstacle), _lib.Obstacle.p1.offset, _lib.Obstacle.radius.offset,
+                 _lib.Obstacle.q.offset]
     from hippolbm_amd import batch as bt
     sd = [int(v) for v in lines[3].split()]
     S = bt.SolveDesc
@@ -85,3 +87,6 @@ def test_enums_match_header():
         assert re.search(rf"= {val},?\s*/\* \"{name}\"", txt), name
     for name, val in _lib.PLANE.items():
         assert re.search(rf"HLBM_{name.upper()} = {val}\b", txt), name
+    for name, val in (("START", -1), ("PRE_STREAM", 0), ("POST_STREAM", 1), ("BOUNDARY", 2)):
+        assert re.search(rf"HLBM_STAGE_{name} = {val}\b", txt), name
+    assert sorted(_lib.STAGE.values()) == [-1, 0, 1, 2]

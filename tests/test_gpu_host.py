@@ -89,3 +89,19 @@ def test_quadric_deck_runs_through_the_operator_layer(tmp_path):
     assert step == d.end_iteration
     assert abs(s - o[0]) <= 1e-12 * abs(o[0])
     assert vmin == o[1] and vmax == o[2]
+
+
+def test_start_iteration_deck_runs_through_the_operator_layer(tmp_path):
+    from test_gpu_parity import START_DECK
+    deck = tmp_path / "start.msp"
+    deck.write_text(START_DECK)
+    r = run([str(deck), "--quiet", "--no-output", "--pattern", "aa"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    step, s, vmin, vmax = final_state(r.stdout)
+    d = dk.loads_msp(START_DECK, "start")
+    ref = orc.OracleRun(d)
+    ref.step(d.end_iteration)
+    o = ref.final_state()
+    assert step == d.end_iteration
+    assert abs(s - o[0]) <= 1e-12 * abs(o[0])
+    assert vmin == o[1] and vmax == o[2]

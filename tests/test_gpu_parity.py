@@ -333,6 +333,72 @@ def test_quadric_obstacles_bitwise(pattern):
     sim.dom.close()
 
 
+# start_iteration: set_distribution on a slab at the start of every step
+# (example/ball.msp:23-26), an MRT channel with a ball off the periodic faces
+START_DECK = """
+do_domain:
+  - domain:
+     bounds: [[0,0,0],[0.1,0.03,0.04]]
+     cell_dims: [50, 15, 20]
+     periodic: [false, true, true]
+set_lbm_parameters:
+  - lbm_parameters:
+     nuth: 1e-4
+set_obstacles:
+  - register_solid_ball:
+     id: 0
+     center: [0.03, 0.015, 0.02]
+     radius: 0.005
+set_distributions:
+  - set_distribution:
+     value: 1.0
+     bounds: [[0,0,0],[0.1,0.03,0.04]]
+start_iteration:
+  - set_distribution:
+     value: 1.2
+     bounds: [[0,0,0.0], [0.004,0.03,0.04]]
+collision: mrt
+pre_stream_bcs:
+  - wall_bounce_back
+global:
+   simulation_end_iteration: 40
+   simulation_print_log_freq: -1
+"""
+
+
+@pytest.mark.parametrize("pattern", ["ab", "aa"])
+def test_start_iteration_set_distribution_bitwise(pattern):
+    d = dk.loads_msp(START_DECK, "start")
+    assert len(d.start_iteration) == 1
+    say("start_iteration", pattern)
+    ref = orc.OracleRun(d)
+    sim = Simulation(d, storage="f64", pattern=pattern)
+    for n in (1, 39):  # one step, then a multi-step call
+        ref.step(n)
+        sim.run(n)
+        nbad, maxdiff, _ = compare(sim, ref, fluid_only=True)
+        assert nbad == 0, (n, nbad, maxdiff)
+    # the slab drives a density wave down the channel
+    rho, _ = sim.dom.macro()
+    assert rho[:, :, 1].mean() > 1.05 and abs(rho[:, :, 45].mean() - 1.0) < 0.05
+    sim.dom.close()
+
+
+def test_start_iteration_reduced_precision_tracks_fp64():
+    d = dk.loads_msp(START_DECK, "start")
+    ref = Simulation(d, storage="f64")
+    lo = Simulation(d, storage="f32")
+    ref.run(40)
+    lo.run(40)
+    r64, u64 = ref.dom.macro()
+    r32, u32 = lo.dom.macro()
+    fluid = ref.dom.get_flags() == -1
+    assert np.abs(r32 - r64)[fluid].max() < 1e-5
+    assert np.linalg.norm((u32 - u64)[:, fluid]) <= 1e-4 * np.linalg.norm(u64[:, fluid])
+    ref.dom.close()
+    lo.dom.close()
+
+
 def test_outflow_after_an_operator_on_its_interior_layer_is_rejected():
     d = _channel_deck(20, 10, 10, 0, 1, "bgk")
     d.boundary_conditions.insert(0, ("lid_driven_cavity", {"U": (0.01, 0.0, 0.0), "regions": ["plan_xy_l"]}))
