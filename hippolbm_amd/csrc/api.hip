@@ -558,6 +558,20 @@ int hlbm_set_distribution(hlbm_domain* h, double value, const double* bounds) {
   return HLBM_OK;
 }
 
+int hlbm_init_double_couette(hlbm_domain* h, int32_t dim, const double U[3]) {
+  if (!h || !U) return fail(HLBM_ERR_ARG, "NULL argument");
+  if (dim < 0 || dim > 2) return fail(HLBM_ERR_ARG, "dim must be 0 (x), 1 (y) or 2 (z)");
+  int rc = materialize(h, 0);
+  if (rc) return rc;
+  const int n = dim == 0 ? h->nx : (dim == 1 ? h->ny : h->nz);
+  // dU = Uc / (0.5 * (domain_size[dim] - 1)) (init_double_couette.cu:85-107)
+  double dU[3];
+  for (int d = 0; d < 3; d++) dU[d] = U[d] / (0.5 * (n - 1));
+  HIP_TRY(launch_double_couette(h->d.storage, h->buf[h->cur], h->stride, h->nx, h->ny, h->nz, dim, dU, U, 0));
+  HIP_TRY(hipDeviceSynchronize());
+  return HLBM_OK;
+}
+
 static int ensure_tmp(hlbm_domain* h) {
   if (!h->tmp) HIP_TRY(hipMalloc((void**)&h->tmp, (size_t)h->N * 8));
   return HLBM_OK;

@@ -67,6 +67,56 @@ __global__ void k_set_dist(void* fv, uint64_t stride, int nx, int ny, int lo0, i
   }
 }
 
+// init_double_couette (src/prepro/include/hippoLBM/prepro/double_couette.hpp:26-45):
+// on every node, u = U - dU * (node coordinate along dim), f_i = w_i * feq(1, u)
+// in the reference's evaluation order.
+template <class S>
+__global__ void k_double_couette(void* fv, uint64_t stride, int nx, int ny, int nz, int dim, double dUx, double dUy,
+                                 double dUz, double Ux, double Uy, double Uz) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= nx || y >= ny || z >= nz) return;
+  const size_t idx = ((size_t)z * ny + y) * nx + x;
+  const double value = dim == 0 ? x : (dim == 1 ? y : z);
+  const double ux = Ux - dUx * value, uy = Uy - dUy * value, uz = Uz - dUz * value;
+  const double u_squ = ux * ux + uy * uy + uz * uz;
+  S* f = (S*)fv;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    const double eu = ux * CX(i) + uy * CY(i) + uz * CZ(i);
+    const double v = 1. * WD(i) * (1. + 3. * eu + 4.5 * eu * eu - 1.5 * u_squ);
+    if constexpr (StoreOps<S>::shifted)
+      StoreOps<S>::st(f, (size_t)i * stride + idx, (float)(v - WD(i)));
+    else
+      StoreOps<S>::st(f, (size_t)i * stride + idx, v);
+  }
+}
+
+template <class S>
+static hipError_t double_couette_t(void* f, uint64_t stride, int nx, int ny, int nz, int dim, const double* dU,
+                                   const double* U, hipStream_t s) {
+  dim3 b(64, 4, 1), g((nx + 63) / 64, (ny + 3) / 4, nz);
+  hipLaunchKernelGGL((k_double_couette<S>), g, b, 0, s, f, stride, nx, ny, nz, dim, dU[0], dU[1], dU[2], U[0], U[1],
+                     U[2]);
+  return hipGetLastError();
+}
+
+hipError_t launch_double_couette(int storage, void* f, uint64_t stride, int nx, int ny, int nz, int dim,
+                                 const double* dU, const double* U, hipStream_t s) {
+  switch (storage) {
+    case 0:
+      return double_couette_t<double>(f, stride, nx, ny, nz, dim, dU, U, s);
+    case 1:
+      return double_couette_t<float>(f, stride, nx, ny, nz, dim, dU, U, s);
+    case 2:
+      return double_couette_t<_Float16>(f, stride, nx, ny, nz, dim, dU, U, s);
+    case 3:
+      return double_couette_t<bf16s>(f, stride, nx, ny, nz, dim, dU, U, s);
+  }
+  return hipErrorInvalidValue;
+}
+
 // FP64 plane -> storage plane (adds the shift)
 template <class S>
 __global__ void k_from_f64(void* fv, const double* src, uint32_t n, int q) {

@@ -61,6 +61,8 @@ hipError_t launch_build_bbm(const uint8_t* flags, int nx, int ny, int nz, const 
                             hipStream_t s);
 hipError_t launch_set_distribution(int storage, void* f, uint64_t stride, int nx, int ny, const int* lo,
                                    const int* hi, double value, hipStream_t s);
+hipError_t launch_double_couette(int storage, void* f, uint64_t stride, int nx, int ny, int nz, int dim,
+                                 const double* dU, const double* U, hipStream_t s);
 hipError_t launch_convert(int storage, bool to_store, void* fplane, double* tmp, uint32_t n, int q, hipStream_t s);
 hipError_t launch_macro(int storage, const void* f, uint64_t stride, uint32_t N, const uint8_t* flags, double* m0,
                         double* m1, hipStream_t s);

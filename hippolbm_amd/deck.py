@@ -254,6 +254,15 @@ def from_dict(doc: dict, name: str = "") -> Deck:
             if "bounds" in v:
                 sd["bounds"] = _aabb(v["bounds"])
             d.set_distributions.append(sd)
+        elif k == "init_double_couette":
+            dim = str(v.get("dimension", ""))
+            if dim not in ("X", "Y", "Z"):
+                raise DeckError('[init_double_couette] Please, select a valid dimension "X", "Y", or "Z".')
+            if "U" not in v:
+                raise DeckError("init_double_couette: the U slot is required")
+            d.set_distributions.append({"kind": "double_couette", "dim": "XYZ".index(dim), "U": _vec(v["U"])})
+        else:
+            raise DeckError(f"unsupported set_distributions operator {k!r}")
     # start_iteration (config_compute_loop.msp:27): operators at the start of every step
     for k, v in _items(doc.get("start_iteration")):
         if k == "set_distribution":

@@ -136,6 +136,11 @@ class Domain:
             b = np.array(list(bounds[0]) + list(bounds[1]), dtype=np.float64)
             check("hlbm_set_distribution", lib().hlbm_set_distribution(self.h, float(value), _dp(b)))
 
+    def init_double_couette(self, dim, U):
+        """init_double_couette with U in LBM units (divided by celerity)."""
+        u = np.ascontiguousarray(U, dtype=np.float64)
+        check("hlbm_init_double_couette", lib().hlbm_init_double_couette(self.h, int(dim), _dp(u)))
+
     def _nall(self):
         return (self.nodes[0] + 4) * (self.nodes[1] + 4) * (self.nodes[2] + 4)
 
@@ -258,7 +263,10 @@ class Simulation:
         self.dom.init_obst()
         self.dom.set_distribution(1.0)
         for sd in deck.set_distributions:
-            self.dom.set_distribution(sd.get("value", 1.0), sd.get("bounds"))
+            if sd.get("kind") == "double_couette":
+                self.dom.init_double_couette(sd["dim"], [u / p["celerity"] for u in sd["U"]])
+            else:
+                self.dom.set_distribution(sd.get("value", 1.0), sd.get("bounds"))
         self.dom.add_obstacles(deck.obstacles_in_id_order())
         if init_f is not None:
             self.dom.set_f(init_f)

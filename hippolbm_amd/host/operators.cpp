@@ -184,6 +184,25 @@ struct SetDistribution : OperatorNode {
   }
 };
 
+// ---- init_double_couette (src/prepro/init_double_couette.cu:71-120) ----------
+struct InitDoubleCouette : OperatorNode {
+  std::vector<double> U;
+  std::string dimension;
+  explicit InitDoubleCouette(const msp::Node& a) {
+    if (!a.find("U") || !a.find("dimension")) throw OperatorError("init_double_couette: U and dimension are required");
+    U = vec3(a.at("U"));
+    dimension = a.at("dimension").text;
+  }
+  void execute(Slots& s) override {
+    const int dim = dimension == "X" ? 0 : dimension == "Y" ? 1 : dimension == "Z" ? 2 : -1;
+    if (dim < 0) throw OperatorError("[init_double_couette] Please, select a valid dimension \"X\", \"Y\", or \"Z\".");
+    const double Uc[3] = {U[0] / s.params.celerity, U[1] / s.params.celerity, U[2] / s.params.celerity};
+    if (dim == 2 && !s.quiet) std::printf("Prepro double couette starting ... dim Z\n");
+    check(hlbm_init_double_couette(s.dom, dim, Uc), "init_double_couette");
+    if (dim == 2 && !s.quiet) std::printf("Prepro double couette ending ... dim Z \n");
+  }
+};
+
 // ---- obstacles (src/obstacle/register_obstacles.cpp, update_obstacles.cu:32-72)
 struct InitObstacles : OperatorNode {
   void execute(Slots& s) override { s.obstacles.clear(); }
@@ -514,6 +533,7 @@ struct Registrar {
     f->register_factory("define_grid_3dq19", plain<DefineGrid>());
     f->register_factory("init_obst", plain<InitObst>());
     f->register_factory("set_distribution", with_args<SetDistribution>());
+    f->register_factory("init_double_couette", with_args<InitDoubleCouette>());
     f->register_factory("init_obstacles", plain<InitObstacles>());
     f->register_factory("register_solid_ball", with_args<RegisterBall>());
     f->register_factory("register_solid_wall", with_args<RegisterWall>());

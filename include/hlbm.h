@@ -169,6 +169,12 @@ int hlbm_get_flags(hlbm_domain* h, int32_t* obst, int32_t layout);
  * nodes of the box bounds = {xmin,ymin,zmin,xmax,ymax,zmax}
  * (src/grid/set_distribution.cu:69-109). */
 int hlbm_set_distribution(hlbm_domain* h, double value, const double* bounds);
+/* init_double_couette (src/prepro/init_double_couette.cu:80-120,
+ * src/prepro/include/hippoLBM/prepro/double_couette.hpp:26-45): every node
+ * gets the equilibrium at rho = 1 of u = U - dU * k, k the node coordinate
+ * along dim (0 x, 1 y, 2 z) and dU = U / (0.5 * (n_dim - 1)); U in LBM units
+ * (the operator's U divided by celerity). */
+int hlbm_init_double_couette(hlbm_domain* h, int32_t dim, const double U[3]);
 /* Populations in FP64 (converted from the storage type; the shifted storage
  * adds w_i back).  get writes the current state f^T as the reference holds it
  * after T time steps (post-stream, post-BC); in reference layout the periodic

@@ -356,6 +356,30 @@ void orc_update_obstacle(void* h, int kind, const double* p0, const double* p1, 
       }
 }
 
+// init_double_couette, src/prepro/init_double_couette.cu:80-120 and
+// src/prepro/include/hippoLBM/prepro/double_couette.hpp:26-45: on the Real
+// box, u = U - dU * (global coordinate along dim), dU = U / (0.5 * (n - 1)),
+// f_i = 1. * w_i * (1. + 3. eu + 4.5 eu eu - 1.5 u.u).  U is in LBM units.
+void orc_init_double_couette(void* h, int dim, const double* U) {
+  Grid* g = (Grid*)h;
+  const int n = g->n[dim];
+  double dU[3];
+  for (int d = 0; d < 3; d++) dU[d] = U[d] / (0.5 * (n - 1));
+  for (int k = GHOST; k < GHOST + g->n[2]; k++)
+    for (int j = GHOST; j < GHOST + g->n[1]; j++)
+      for (int i = GHOST; i < GHOST + g->n[0]; i++) {
+        const int c[3] = {i, j, k};
+        const double value = c[dim] - GHOST;
+        const double u[3] = {U[0] - dU[0] * value, U[1] - dU[1] * value, U[2] - dU[2] * value};
+        const double u_squ = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        const size_t idx = g->idx(i, j, k);
+        for (int q = 0; q < Q; q++) {
+          const double eu = u[0] * EX[q] + u[1] * EY[q] + u[2] * EZ[q];
+          g->f[(size_t)q * g->N + idx] = 1. * W(q) * (1. + 3. * eu + 4.5 * eu * eu - 1.5 * u_squ);
+        }
+      }
+}
+
 // Quadric obstacle (register_quadrics.cpp:72-84, obstacle.hpp:107-138): the
 // candidate box is Quadric::covered() = [-1e6, 1e6]^3 truncated by int(b/dx)
 // (clamped to the int range) and restricted to the Extend box; solid where

@@ -68,6 +68,7 @@ def lib():
             "orc_get_node_populations": (None, [P, DP]),
             "orc_update_obstacle": (None, [P, I, DP, DP, D]),
             "orc_update_quadric": (None, [P, DP]),
+            "orc_init_double_couette": (None, [P, I, DP]),
             "orc_macro_variables": (None, [P]),
             "orc_bgk": (None, [P, D, D, D, D]),
             "orc_mrt": (None, [P, D, D, D, D, DP]),
@@ -170,6 +171,10 @@ class Oracle:
         b = np.asarray(p1, dtype=np.float64)
         self.L.orc_update_obstacle(self.h, {"ball": 0, "wall": 1}[kind], _dp(a), _dp(b), float(r))
 
+    def init_double_couette(self, dim, U):
+        u = np.ascontiguousarray(U, dtype=np.float64)
+        self.L.orc_init_double_couette(self.h, int(dim), _dp(u))
+
     def update_quadric(self, q):
         a = np.ascontiguousarray(q, dtype=np.float64).reshape(16)
         self.L.orc_update_quadric(self.h, _dp(a))
@@ -242,7 +247,10 @@ class OracleRun:
         o.init_obst()
         o.set_distribution(1.0, None, do_update=False)
         for sd in deck.set_distributions:
-            o.set_distribution(sd.get("value", 1.0), sd.get("bounds"), bool(sd.get("do_update", False)))
+            if sd.get("kind") == "double_couette":
+                o.init_double_couette(sd["dim"], [u / self.p["celerity"] for u in sd["U"]])
+            else:
+                o.set_distribution(sd.get("value", 1.0), sd.get("bounds"), bool(sd.get("do_update", False)))
         for ob in deck.obstacles_in_id_order():
             if ob["kind"] == "ball":
                 o.update_obstacle("ball", ob["center"], r=ob["radius"])

@@ -105,3 +105,20 @@ def test_start_iteration_deck_runs_through_the_operator_layer(tmp_path):
     assert step == d.end_iteration
     assert abs(s - o[0]) <= 1e-12 * abs(o[0])
     assert vmin == o[1] and vmax == o[2]
+
+
+def test_double_couette_deck_runs_through_the_operator_layer(tmp_path):
+    from test_gpu_parity import DOUBLE_COUETTE_DECK
+    deck = tmp_path / "dc.msp"
+    deck.write_text(DOUBLE_COUETTE_DECK)
+    r = run([str(deck), "--no-output"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Prepro double couette starting ... dim Z" in r.stdout
+    r = run([str(deck), "--quiet", "--no-output"], tmp_path)
+    step, s, vmin, vmax = final_state(r.stdout)
+    d = dk.loads_msp(DOUBLE_COUETTE_DECK, "dc")
+    ref = orc.OracleRun(d)
+    ref.step(d.end_iteration)
+    o = ref.final_state()
+    assert abs(s - o[0]) <= 1e-12 * abs(o[0])
+    assert vmin == o[1] and vmax == o[2]

@@ -399,6 +399,55 @@ def test_start_iteration_reduced_precision_tracks_fp64():
     lo.dom.close()
 
 
+# init_double_couette (example/lbm_prepro_double_couette.msp at 31^3)
+DOUBLE_COUETTE_DECK = """
+do_domain:
+  - domain:
+     bounds:    [ [0,0,0],[0.03,0.03,0.03] ]
+     cell_dims: [ 30 , 30 , 30 ]
+     periodic:  [ true, true, false ]
+set_distributions:
+  - init_double_couette:
+     U: [0.001,0,0]
+     dimension: Z
+set_lbm_parameters:
+  - lbm_parameters:
+     Fext: [0,0,0]
+     nuth: 1e-2
+     celerity: 1.0
+boundary_conditions:
+  - neumann:
+     U: [-0.001,0,0]
+     regions: [plan_xy_l]
+  - neumann:
+     U: [0.001,0,0]
+     regions: [plan_xy_0]
+global:
+   simulation_end_iteration: 20
+   simulation_print_log_freq: -1
+"""
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32"])
+def test_init_double_couette(storage):
+    d = dk.loads_msp(DOUBLE_COUETTE_DECK, "double_couette")
+    say("double couette", storage)
+    ref = orc.OracleRun(d)
+    sim = Simulation(d, storage=storage)
+    ref.o.macro_variables()
+    rho, u = sim.dom.macro()
+    if storage == "f64":
+        assert np.array_equal(sim.dom.get_f(), ref.o.node_f())
+        ref.step(20)
+        sim.run(20)
+        nbad, maxdiff, _ = compare(sim, ref)
+        assert nbad == 0, (nbad, maxdiff)
+    else:
+        uo = ref.o.real(ref.o.m1())
+        assert np.abs(u - uo).max() < 1e-9 and np.abs(rho - 1.0).max() < 1e-6
+    sim.dom.close()
+
+
 def test_outflow_after_an_operator_on_its_interior_layer_is_rejected():
     d = _channel_deck(20, 10, 10, 0, 1, "bgk")
     d.boundary_conditions.insert(0, ("lid_driven_cavity", {"U": (0.01, 0.0, 0.0), "regions": ["plan_xy_l"]}))

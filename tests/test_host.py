@@ -50,9 +50,9 @@ def test_reader_matches_yaml(path):
 
 def test_unknown_operator_is_refused(tmp_path):
     deck = tmp_path / "bad.msp"
-    deck.write_text(open(DECK_FILES[0]).read() + "\nboundary_conditions:\n  - init_double_couette\n")
+    deck.write_text(open(DECK_FILES[0]).read() + "\nboundary_conditions:\n  - write_vtk_slices\n")
     r = subprocess.run([RUN, str(deck)], capture_output=True, text=True, cwd=tmp_path)
-    assert r.returncode != 0 and "unsupported operator 'init_double_couette'" in r.stdout
+    assert r.returncode != 0 and "unsupported operator 'write_vtk_slices'" in r.stdout
 
 
 def test_required_analysis_slot_is_checked_before_device_work(tmp_path):

@@ -153,6 +153,26 @@ def test_quadric_rasterization_brute_force():
     assert (o.obst()[2:-2, 2:-2, :] == -2).sum(axis=2).max() == 3  # nodes x = -2, -1, 0 (ghosts included)
 
 
+def test_init_double_couette_is_the_linear_shear_equilibrium():
+    # rho = 1 and u = U (1 - 2k/(n-1)) along the chosen axis, to round-off
+    n = (6, 7, 11)
+    U = (0.01, -0.002, 0.0)
+    for dim in range(3):
+        o = orc.Oracle(n, (True, True, False), 1.0)
+        o.init_obst()
+        o.set_distribution(1.0)
+        o.init_double_couette(dim, U)
+        o.macro_variables()
+        rho, u = o.real(o.m0()), o.real(o.m1())
+        k = np.indices(rho.shape)[2 - dim]
+        assert np.abs(rho - 1.0).max() < 1e-15
+        for c in range(3):
+            exact = U[c] * (1.0 - 2.0 * k / (n[dim] - 1))
+            assert np.abs(u[c] - exact).max() < 1e-15
+        # the ghost layers keep the previous state (Real box only)
+        assert np.all(o.m1()[:, :2] == 0.0) if dim == 2 else True
+
+
 def test_quadric_deck_parses_to_obstacles():
     txt = open(os.path.join(DECKS, "regression_lbm_cavite.msp")).read()
     txt += """
