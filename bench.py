@@ -184,6 +184,70 @@ def precision_sweep(steps=10000, which=("couette", "poiseuille")):
     return out
 
 
+def output_operators(n=512, reps=5):
+    """The output path: plane_velocity_profile per axis and plot_line_velocity on the
+    Couette n^3 FP64 state (device reductions, timed per call including the copy
+    back), and write_paraview through hlbm_run on a 128^3 deck (host text)."""
+    import subprocess
+    import tempfile
+    dom = couette_domain(n, "f64")
+    dom.step(20, write_macro=True)
+    dom.synchronize()
+    N = n ** 3
+    out = {"nodes": [n, n, n], "bytes_per_node": 3 * 8 + 1}
+    for dim, name in enumerate("xyz"):
+        dom.plane_profile(dim)  # warm-up
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            dom.plane_profile(dim, 1.0)
+        ms = 1e3 * (time.perf_counter() - t0) / reps
+        out[f"plane_profile_{name}_ms"] = ms
+        out[f"plane_profile_{name}_gbs"] = 25.0 * N / (ms * 1e-3) / 1e9
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        dom.line_velocity((n // 2, n // 2, 0), 2, n)
+    out["line_velocity_ms"] = 1e3 * (time.perf_counter() - t0) / reps
+    dom.close()
+    m = 128
+    deck = f"""
+do_domain:
+  - domain:
+     bounds: [[0,0,0],[{m - 1},{m - 1},{m - 1}]]
+     cell_dims: [{m - 1},{m - 1},{m - 1}]
+     periodic: [true, true, false]
+set_lbm_parameters:
+  - lbm_parameters:
+     tau: 0.8
+     nuth: 0.1
+pre_stream_bcs:
+  - pre_bounce_back
+  - cavity_z_l:
+     U: [0.05, 0, 0]
+post_stream_bcs:
+  - post_bounce_back
+global:
+   simulation_end_iteration: 10
+   simulation_print_log_freq: -1
+"""
+    run = os.path.join(ROOT, "hippolbm_amd", "hlbm_run")
+    with tempfile.TemporaryDirectory() as tmp:
+        path = os.path.join(tmp, "io.msp")
+        with open(path, "w") as fh:
+            fh.write(deck)
+        walls = []
+        for extra in (["--no-output"], []):
+            t0 = time.perf_counter()
+            subprocess.run([run, path, "--quiet"] + extra, check=True, capture_output=True, cwd=tmp)
+            walls.append(time.perf_counter() - t0)
+        pv = os.path.join(tmp, "hippoLBMOutputDir", "ParaviewOutput")
+        size = sum(os.path.getsize(os.path.join(dp, f)) for dp, _, fs in os.walk(pv) for f in fs)
+    dumps = 2  # first time step + the final write
+    out["paraview"] = {"nodes": [m, m, m], "dumps": dumps, "bytes_per_dump": size / dumps,
+                       "s_per_dump": (walls[1] - walls[0]) / dumps,
+                       "mb_per_s": size / 1e6 / max(walls[1] - walls[0], 1e-9)}
+    return out
+
+
 def copy_bandwidth():
     import ctypes
     from hippolbm_amd import _lib
@@ -253,6 +317,7 @@ def main():
     ap.add_argument("--precision", type=int, default=0, metavar="STEPS",
                     help="run only the precision sweep (Couette 512^3, Poiseuille 512x256x256) for STEPS steps")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
+    ap.add_argument("--io", action="store_true", help="run only the output-operator measurements")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -261,6 +326,9 @@ def main():
     if args.copy:
         print(json.dumps({"copy_kernel_gbs": copy_bandwidth(), "bytes_read_per_launch": 4 << 30,
                           "bytes_written_per_launch": 4 << 30}), flush=True)
+        return 0
+    if args.io:
+        print(json.dumps({"output_operators": output_operators(args.size)}), flush=True)
         return 0
     if args.precision:
         print(json.dumps({"precision_sweep": precision_sweep(args.precision)}), flush=True)

@@ -279,14 +279,17 @@ __global__ void __launch_bounds__(256)
   }
   double sum = 0.0, vmin = 1.7976931348623157e308, vmax = -1.7976931348623157e308;
   int cnt = 0;
+  // every load is issued (walls are selected out afterwards), so the unrolled
+  // iterations keep their loads in flight together
+#pragma unroll 8
   for (int b = 0; b < nb; b++, idx += step) {
-    if (flags[idx]) continue;
+    const bool fluid = flags[idx] == 0;
     const double ux = scale * m1[idx], uy = scale * m1[(size_t)N + idx], uz = scale * m1[2 * (size_t)N + idx];
     const double v = sqrt(ux * ux + uy * uy + uz * uz);
-    sum += v;
-    cnt += 1;
-    vmin = fmin(vmin, v);
-    vmax = fmax(vmax, v);
+    sum += fluid ? v : 0.0;
+    cnt += fluid ? 1 : 0;
+    vmin = fluid ? fmin(vmin, v) : vmin;
+    vmax = fluid ? fmax(vmax, v) : vmax;
   }
   ProfileStat o;
   o.sum = sum;
