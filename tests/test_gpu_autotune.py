@@ -8,7 +8,7 @@ import pytest
 
 from hippolbm_amd.engine import Simulation
 
-from test_gpu_parity import _couette_deck, _porous_deck, seeded_equilibrium
+from test_gpu_parity import _channel_deck, _couette_deck, _porous_deck, seeded_equilibrium
 
 pytestmark = pytest.mark.gpu
 
@@ -47,6 +47,22 @@ def test_every_launch_shape_gives_the_same_populations(shape):
     b.dom.set_launch(*shape)
     a.run(13)
     b.run(13)
+    assert np.array_equal(a.dom.get_f(), b.dom.get_f())
+    a.dom.close()
+    b.dom.close()
+
+
+@pytest.mark.parametrize("pattern", ["ab", "aa"])
+@pytest.mark.parametrize("shape", [((64, 1), 1, 0), ((64, 4), 2, 1), ((256, 1), 2, 0)])
+def test_launch_shapes_with_boundary_program_planes(shape, pattern):
+    # density inlet + outflow planes (the boundary kernel) beside bulk tiles of
+    # one or two rows per thread
+    d = _channel_deck(40, 18, 14, 4, 3, "mrt")
+    a = Simulation(d, storage="f64", pattern=pattern)
+    b = Simulation(d, storage="f64", pattern=pattern)
+    b.dom.set_launch(*shape)
+    a.run(11)
+    b.run(11)
     assert np.array_equal(a.dom.get_f(), b.dom.get_f())
     a.dom.close()
     b.dom.close()
