@@ -126,6 +126,35 @@ struct LbmParametersOp : OperatorNode {
   }
 };
 
+// ---- print_domain (src/grid/print_domain.cpp:58-83) -------------------------
+// Informational.  Boxes in the reference's global (Real) and local (Extend)
+// coordinates of a single rank; the AABB is written as [(x,y,z),(x,y,z)]
+// (Onika's stream format for it is not in the reference tree).
+struct PrintDomain : OperatorNode {
+  void execute(Slots& s) override {
+    if (s.quiet) return;
+    if (!s.has_domain) throw OperatorError("print_domain: the domain is not defined");
+    const int* n = s.nodes;
+    std::printf("=================================\n");
+    std::printf("== Domain size: [(%g,%g,%g),(%g,%g,%g)]\n", s.bounds[0][0], s.bounds[0][1], s.bounds[0][2],
+                s.bounds[1][0], s.bounds[1][1], s.bounds[1][2]);
+    std::printf("== Dx: %g\n", s.dx);
+    std::printf("== Grid size: (LX: %d, LY: %d, LZ %d)\n", n[0], n[1], n[2]);
+    std::printf("=================================\n");
+    std::printf("== Grid info \n= Rank 0 \n= Real Grid:\n");
+    std::printf(" inf_: 0 0 0\n sup_: %d %d %d\n", n[0] - 1, n[1] - 1, n[2] - 1);
+    std::printf("= Extended Grid: \n");
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {  // make_domain.hpp:178-188: All on periodic axes, Real otherwise (local)
+      lo[d] = s.periodic[d] ? 0 : 2;
+      hi[d] = s.periodic[d] ? n[d] + 3 : n[d] + 1;
+    }
+    std::printf(" inf_: %d %d %d\n sup_: %d %d %d\n", lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]);
+    std::printf("=================================\n");
+    std::fflush(stdout);
+  }
+};
+
 // ---- define_grid_3dq19 (src/grid/define_lbm_fields.cpp:58-86) --------------
 struct DefineGrid : OperatorNode {
   void execute(Slots& s) override {
@@ -339,6 +368,28 @@ struct RegisterQuadric : OperatorNode {
   void execute(Slots& s) override { s.obstacles[id] = o; }
 };
 
+// ---- display_obstacles (src/obstacle/display_obstacles.cpp:46-53,
+//      obstacle.hpp:68-71,101-104,139-147) ----------------------------------------
+struct DisplayObstacles : OperatorNode {
+  void execute(Slots& s) override {
+    if (s.quiet) return;
+    for (const auto& kv : s.obstacles) {
+      const hlbm_obstacle& o = kv.second;
+      if (o.kind == 0) {
+        std::printf("Ball center: (%g, %g, %g), radius: %g\n", o.p0[0], o.p0[1], o.p0[2], o.radius);
+      } else if (o.kind == 1) {
+        std::printf("Wall bounds: [(%g, %g, %g), (%g, %g, %g)]\n", o.p0[0], o.p0[1], o.p0[2], o.p1[0], o.p1[1],
+                    o.p1[2]);
+      } else {
+        std::printf("Quadric matrix: \n");
+        for (int i = 0; i < 4; i++)
+          std::printf("%g %g %g %g \n", o.q[4 * i], o.q[4 * i + 1], o.q[4 * i + 2], o.q[4 * i + 3]);
+      }
+    }
+    std::fflush(stdout);
+  }
+};
+
 struct UpdateObstacles : OperatorNode {
   void execute(Slots& s) override {
     std::vector<hlbm_obstacle> v;
@@ -539,6 +590,8 @@ struct Registrar {
     f->register_factory("register_solid_wall", with_args<RegisterWall>());
     f->register_factory("register_quadrics", with_args<RegisterQuadric>());
     f->register_factory("update_obstacles", plain<UpdateObstacles>());
+    f->register_factory("display_obstacles", plain<DisplayObstacles>());
+    f->register_factory("print_domain", plain<PrintDomain>());
     f->register_factory("macro_variables", plain<MacroVariables>());
     f->register_factory("bgk", [](const msp::Node&) { return std::unique_ptr<OperatorNode>(new Collision(HLBM_BGK)); });
     f->register_factory("mrt", [](const msp::Node&) { return std::unique_ptr<OperatorNode>(new Collision(HLBM_MRT)); });
@@ -682,6 +735,7 @@ void Simulation::initialize() {
   run(do_domain, s);
   if (set_lbm_parameters.empty()) set_lbm_parameters.push_back(OperatorNodeFactory::instance()->make("lbm_parameters", msp::Node{}));
   run(set_lbm_parameters, s);
+  PrintDomain().execute(s);
   DefineGrid().execute(s);
   if (s.autotune) {
     hlbm_tune_info info{};

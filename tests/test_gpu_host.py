@@ -60,6 +60,8 @@ def test_log_table_and_aa_pattern(tmp_path):
     r = run([path, "--pattern", "aa", "--steps", "7", "--no-output"], tmp_path)
     assert r.returncode == 0, r.stdout + r.stderr
     lines = r.stdout.splitlines()
+    assert "== Grid size: (LX: 31, LY: 31, LZ 31)" in lines  # print_domain first
+    lines = lines[[l.split()[:1] for l in lines].index(["Step"]):]
     assert lines[0].split() == ["Step", "Time", "Mesh", "Size", "Sum(density)", "min(||V||)", "max(||V||)", "MLUPS"]
     steps = [int(l.split()[0]) for l in lines[1:] if l.strip() and l.split()[0].isdigit()]
     assert steps == list(range(0, 8))
