@@ -107,6 +107,8 @@ class TuneInfo(ctypes.Structure):
         ("npt", ctypes.c_int32),
         ("xcd_remap", ctypes.c_int32),
         ("candidates", ctypes.c_int32),
+        ("variant", ctypes.c_int32),
+        ("relocated", ctypes.c_int32),
         ("pad", ctypes.c_int32),
         ("ms_best", ctypes.c_double),
         ("ms_default", ctypes.c_double),
@@ -159,7 +161,8 @@ def lib():
         "hlbm_ghost_update_ref": (ctypes.c_int, [P, DP]),
         "hlbm_copy_bench": (ctypes.c_int, [ctypes.c_uint64, ctypes.c_int32, P, DP]),
         "hlbm_autotune": (ctypes.c_int, [P, ctypes.c_int32, P, ctypes.POINTER(TuneInfo)]),
-        "hlbm_set_launch": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+        "hlbm_set_launch": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.c_int32]),
         "hlbm_set_collision": (ctypes.c_int, [P, ctypes.c_int32, DP]),
         "hlbm_plane_profile": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_double, DP, I32P, DP, DP]),
         "hlbm_line_velocity": (ctypes.c_int, [P, I32P, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, DP]),

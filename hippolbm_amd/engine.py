@@ -178,12 +178,15 @@ class Domain:
         t = _lib.TuneInfo()
         s = ctypes.c_void_p(stream) if stream else None
         check("hlbm_autotune", lib().hlbm_autotune(self.h, int(reps), s, ctypes.byref(t)))
-        return {"block": (t.block_x, t.block_y), "npt": t.npt, "xcd_remap": t.xcd_remap,
-                "candidates": t.candidates, "ms_best": t.ms_best, "ms_default": t.ms_default}
+        return {"block": (t.block_x, t.block_y), "npt": t.npt, "xcd_remap": t.xcd_remap, "variant": t.variant,
+                "candidates": t.candidates, "relocated": t.relocated, "ms_best": t.ms_best,
+                "ms_default": t.ms_default}
 
-    def set_launch(self, block=(128, 2), npt=1, xcd_remap=1):
+    def set_launch(self, block=(128, 2), npt=1, xcd_remap=1, variant=0):
+        """Launch shape by hand; variant 0 = scalar plane bases + 32-bit offsets,
+        1 = 64-bit per-lane addresses."""
         check("hlbm_set_launch", lib().hlbm_set_launch(self.h, int(block[0]), int(block[1]), int(npt),
-                                                       int(xcd_remap)))
+                                                       int(xcd_remap), int(variant)))
 
     def bytes(self):
         b = ctypes.c_uint64(0)

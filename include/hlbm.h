@@ -294,25 +294,38 @@ double hlbm_clock(void);
 
 /* ---- launch-shape autotuner (no counterpart in the reference) -----------
  * The step kernels take a tile shape (block_x nodes along x by block_y rows),
- * a count of nodes per thread along y (npt, 1 or 2) and an XCD-aware tile
- * order.  hlbm_autotune times every candidate on scratch memory (the domain's
+ * a count of nodes per thread along y (npt, 1 or 2), an XCD-aware tile order
+ * and a bulk-kernel variant: 0 = plane bases in scalar registers with 32-bit
+ * lane offsets and a per-node bounce-back mask (fewest instructions), 1 =
+ * 64-bit per-lane addresses and neighbour flags (its loads schedule more
+ * freely; npt is 1).  hlbm_autotune times every candidate on scratch memory (the domain's
  * populations, phase and step count are left as they were: A/B domains write
  * only the buffer the next step overwrites; AA domains work on a temporary
  * copy), keeps the fastest for this domain and caches it per (dims, storage,
  * collision, pattern, boundary program, device) so later domains of the same
- * shape skip the search (out->candidates == 0).  Results never depend on the
- * shape: every node runs the same arithmetic. */
+ * shape skip the search (out->candidates == 0).  Before the search, an A/B
+ * domain of 256 MiB of populations or more checks its placement: the step speed
+ * depends on where the population buffers land in HBM (up to ~8 %, fixed per
+ * allocation), so up to two fresh buffer pairs are timed and the state moves
+ * to a faster one (out->relocated; needs twice the population memory free for
+ * the duration; HLBM_PLACEMENT=0 turns it off).  Results never depend on the
+ * shape or the placement: every node runs the same arithmetic. */
 typedef struct {
   int32_t block_x, block_y, npt, xcd_remap; /* chosen shape */
   int32_t candidates;                       /* shapes timed; 0 = taken from the cache */
+  int32_t variant;                          /* bulk-kernel variant, 0 or 1 */
+  int32_t relocated;                        /* population buffers moved by the placement check */
   int32_t pad;
   double ms_best;                           /* per step, chosen shape */
   double ms_default;                        /* per step, the default 128x2 / npt 1 / remap */
 } hlbm_tune_info;
 
 int hlbm_autotune(hlbm_domain* h, int32_t reps, void* stream, hlbm_tune_info* out);
-/* Set the shape by hand: block_x*block_y a multiple of 64 and <= 256, npt 1|2. */
-int hlbm_set_launch(hlbm_domain* h, int32_t block_x, int32_t block_y, int32_t npt, int32_t xcd_remap);
+/* Set the shape by hand: block_x*block_y a multiple of 64 and <= 256, npt 1|2,
+ * variant 0|1 (domains above 2^32 bytes per population plane, or with
+ * start_iteration boxes, always take variant 1). */
+int hlbm_set_launch(hlbm_domain* h, int32_t block_x, int32_t block_y, int32_t npt, int32_t xcd_remap,
+                    int32_t variant);
 
 /* ---- measurement helper (no counterpart in the reference) --------------
  * Times `reps` streaming copies of `bytes` bytes between two scratch buffers
