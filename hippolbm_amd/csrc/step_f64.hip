@@ -15,11 +15,19 @@ static hipError_t go(dim3 grid, dim3 block, const StepParams& p, hipStream_t s) 
       if (e != hipSuccess) return e;
     }
   }
+  // the program-plane nodes and the bulk touch disjoint nodes (A/B) or disjoint
+  // slots (AA), so either kernel may run first
+  const bool planes = MODE != MODE_COLLIDE && p.prog_planes;
+  if (planes && p.boundary_first) {
+    hipLaunchKernelGGL((boundary_kernel<double, COLL, MODE>), dim3(boundary_blocks(p)), dim3(256), 0, s, p);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
   if (MODE != MODE_AA_FIX) {  // AA_FIX only touches program-plane nodes
     hipLaunchKernelGGL((step_kernel<double, COLL, MODE, NPT, WIDE>), grid, block, 0, s, p);
     e = hipGetLastError();
   }
-  if (e != hipSuccess || MODE == MODE_COLLIDE || !p.prog_planes) return e;
+  if (e != hipSuccess || !planes || p.boundary_first) return e;
   hipLaunchKernelGGL((boundary_kernel<double, COLL, MODE>), dim3(boundary_blocks(p)), dim3(256), 0, s, p);
   return hipGetLastError();
 }

@@ -103,6 +103,7 @@ struct StepParams {
   double ef[Q];         // c_i . F, evaluated in the reference's order on the host
   double mrt[10];       // s1, s2, s4, s9, s10, s13, s16, weps, wepsj, wxx (mrt_custom)
   ProgEntry prog[MAX_PROG];
+  int32_t boundary_first;      // launch the program-plane kernel before the bulk kernel
   int32_t nset;                // start_iteration set_distribution boxes
   int32_t set_box[MAX_SET][6]; // node box, inclusive: x0 y0 z0 x1 y1 z1
   double set_val[MAX_SET];     // f_i = value * w_i
