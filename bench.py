@@ -28,10 +28,10 @@ HBM_PEAK_GBS = 8000.0
 BYTES = {"f64": 8, "f32": 4, "f16": 2, "bf16": 2}
 
 
-def couette_domain(n, storage, collision="bgk"):
+def couette_domain(n, storage, collision="bgk", pattern="ab"):
     from hippolbm_amd.engine import Domain
     dom = Domain((n, n, n), (True, True, False), 0.8, (0.0, 0.0, 0.0), storage=storage, collision=collision,
-                 dx=1.0)
+                 dx=1.0, pattern=pattern)
     dom.set_bcs([("pre_stream_bcs", "pre_bounce_back", {}), ("pre_stream_bcs", "cavity_z_l", {"U": (0.05, 0, 0)}),
                  ("post_stream_bcs", "post_bounce_back", {})])
     dom.init_obst()
@@ -58,8 +58,8 @@ def time_steps(dom, steps, warmup):
     return ms, wall
 
 
-def measure(n, storage, steps, warmup, collision="bgk", tune=True):
-    dom = couette_domain(n, storage, collision)
+def measure(n, storage, steps, warmup, collision="bgk", tune=True, pattern="ab"):
+    dom = couette_domain(n, storage, collision, pattern)
     tuned = dom.autotune(reps=3) if tune else None
     ms, wall = time_steps(dom, steps, warmup)
     # sanity: the run stayed finite and mass-conserving
@@ -347,6 +347,12 @@ def main():
         print(f"# {dt}: {per[dt]['mlups']:.0f} MLUPS, {per[dt]['achieved_gbs']:.0f} GB/s "
               f"({100 * per[dt]['frac']:.1f}% of 8 TB/s)", file=sys.stderr, flush=True)
     main_r = per.get("f64") or next(iter(per.values()))
+    # the AA pattern (one lattice, in place): same bytes per update, half the footprint
+    aa = {}
+    for dt in ("f64", "f16"):
+        r = measure(n, dt, args.steps, args.warmup, tune=not args.no_tune, pattern="aa")
+        aa[dt] = {"mlups": r["mlups"], "frac": r["frac"], "ms_per_step": r["ms_per_step"], "launch": r["launch"]}
+        print(f"# AA {dt}: {r['mlups']:.0f} MLUPS ({100 * r['frac']:.1f}%)", file=sys.stderr, flush=True)
     copy_gbs = copy_bandwidth()
     porous = porous_channel()
     print(f"# porous channel MRT f64: {porous['mlups']:.0f} MLUPS ({100 * porous['frac']:.1f}%)", file=sys.stderr,
@@ -390,6 +396,7 @@ def main():
                      "aggregate_mlups": served["f32"][2]["mlups"], "frac": served["f32"][2]["frac"],
                      "workload": f"served stream, FP32 storage, Poisson arrivals at {served['f32'][2]['rate_hz']} Hz"}
                     if served else None),
+        "aa_pattern": aa,
         "porous_channel": porous,
         "batched": batched,
         "served": served,

@@ -884,53 +884,88 @@ static int time_shape(hlbm_domain* h, void* scratch, int reps, hipStream_t s, hi
   return HLBM_OK;
 }
 
-// Population placement (A/B domains of 256 MiB of populations and more).  The same kernel on the
-// same data runs up to ~8 % apart depending on where its population buffers
-// land in HBM (measured per allocation; stable for the allocation's lifetime).
-// Up to `tries` fresh pairs of buffers are timed with the current launch
-// shape; the state moves to a pair that is faster by more than 2 %.
+// Population placement (domains with 256 MiB of populations or more).  The
+// same kernel on the same data runs up to ~10 % apart depending on where its
+// population buffers land in HBM (measured per allocation; stable for the
+// allocation's lifetime).  Up to `tries` fresh buffers are timed with the
+// current launch shape; the state moves to one that is faster by more than 2 %.
+// A/B: the step reads the current buffer and writes the other one, which the
+// next step overwrites anyway, so both pairs are timed as they are.  AA: the
+// step works in place, so the current lattice is saved first and restored
+// after its timing, and candidates are timed on copies.
 static int tune_placement(hlbm_domain* h, hipStream_t s, hipEvent_t e0, hipEvent_t e1, int reps, int tries,
                           bool force, int* moved) {
   *moved = 0;
-  if (h->d.pattern == HLBM_PATTERN_AA) return HLBM_OK;
+  const bool aa = h->d.pattern == HLBM_PATTERN_AA;
   const size_t fbytes = (size_t)h->stride * Q * h->esize;
   if (fbytes < ((size_t)256 << 20)) return HLBM_OK;
-  double best = 0.0;
-  int rc = time_shape(h, nullptr, reps, s, e0, e1, &best);
-  for (int t = 0; t < tries && rc == HLBM_OK; t++) {
-    size_t fr = 0, tot = 0;
-    if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr < 2 * fbytes + ((size_t)4 << 30)) break;
-    void* nb[2] = {nullptr, nullptr};
-    if (hipMalloc(&nb[0], fbytes) != hipSuccess) break;
-    if (hipMalloc(&nb[1], fbytes) != hipSuccess) {
-      (void)hipFree(nb[0]);
-      break;
-    }
-    // either buffer may hold state, depending on the phase
-    if (hipMemcpyAsync(nb[0], h->buf[0], fbytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(nb[1], h->buf[1], fbytes, hipMemcpyDeviceToDevice, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
-      (void)hipFree(nb[0]);
-      (void)hipFree(nb[1]);
+  const int nbuf = aa ? 1 : 2;
+  auto copy = [&](void* dst, const void* src) {
+    return hipMemcpyAsync(dst, src, fbytes, hipMemcpyDeviceToDevice, s) == hipSuccess;
+  };
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr < 2 * fbytes + ((size_t)4 << 30)) return HLBM_OK;
+  void* save = nullptr;  // AA: the state while the lattice itself is timed
+  if (aa) {
+    if (hipMalloc(&save, fbytes) != hipSuccess) return HLBM_OK;
+    if (!copy(save, h->buf[h->cur])) {
+      (void)hipFree(save);
       return fail(HLBM_ERR_HIP, "autotune: placement copy");
     }
-    void* old[2] = {h->buf[0], h->buf[1]};
-    h->buf[0] = nb[0];
-    h->buf[1] = nb[1];
+  }
+  double best = 0.0;
+  int rc = time_shape(h, aa ? h->buf[h->cur] : nullptr, reps, s, e0, e1, &best);
+  if (rc == HLBM_OK && aa && (!copy(h->buf[h->cur], save) || hipStreamSynchronize(s) != hipSuccess))
+    rc = fail(HLBM_ERR_HIP, "autotune: placement restore");
+  for (int t = 0; t < tries && rc == HLBM_OK; t++) {
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess || fr < (size_t)(nbuf + 1) * fbytes + ((size_t)4 << 30)) break;
+    void* nb[2] = {nullptr, nullptr};
+    bool ok = true;
+    for (int k = 0; k < nbuf && ok; k++) ok = hipMalloc(&nb[k], fbytes) == hipSuccess;
+    if (!ok) {
+      for (void* q : nb)
+        if (q) (void)hipFree(q);
+      break;
+    }
+    // A/B: either buffer may hold state, depending on the phase
+    for (int k = 0; k < nbuf && ok; k++) ok = copy(nb[aa ? 0 : k], aa ? save : h->buf[k]);
+    if (!ok || hipStreamSynchronize(s) != hipSuccess) {
+      for (void* q : nb)
+        if (q) (void)hipFree(q);
+      rc = fail(HLBM_ERR_HIP, "autotune: placement copy");
+      break;
+    }
     double ms = 0.0;
-    rc = time_shape(h, nullptr, reps, s, e0, e1, &ms);
+    void* old[2] = {h->buf[0], h->buf[1]};
+    if (aa) {
+      rc = time_shape(h, nb[0], reps, s, e0, e1, &ms);
+      if (rc == HLBM_OK && (!copy(nb[0], save) || hipStreamSynchronize(s) != hipSuccess))
+        rc = fail(HLBM_ERR_HIP, "autotune: placement restore");
+    } else {
+      h->buf[0] = nb[0];
+      h->buf[1] = nb[1];
+      rc = time_shape(h, nullptr, reps, s, e0, e1, &ms);
+      h->buf[0] = old[0];
+      h->buf[1] = old[1];
+    }
     if (rc == HLBM_OK && (force || ms < 0.98 * best)) {
       best = ms;
       (*moved)++;
-      (void)hipFree(old[0]);
-      (void)hipFree(old[1]);
+      if (aa) {
+        (void)hipFree(h->buf[h->cur]);
+        h->buf[h->cur] = nb[0];
+      } else {
+        (void)hipFree(old[0]);
+        (void)hipFree(old[1]);
+        h->buf[0] = nb[0];
+        h->buf[1] = nb[1];
+      }
     } else {
-      h->buf[0] = old[0];
-      h->buf[1] = old[1];
-      (void)hipFree(nb[0]);
-      (void)hipFree(nb[1]);
+      for (void* q : nb)
+        if (q) (void)hipFree(q);
     }
   }
+  if (save) (void)hipFree(save);
   return rc;
 }
 
@@ -942,6 +977,27 @@ int hlbm_autotune(hlbm_domain* h, int32_t reps, void* stream, hlbm_tune_info* ou
     const int rc = ensure_bbm(h, s);
     if (rc) return rc;
   }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return fail(HLBM_ERR_HIP, "autotune events");
+  }
+  auto done = [&](int rc) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+  };
+  // placement first: it belongs to this domain's allocation, not to its shape
+  int moved = 0;
+  {
+    // HLBM_PLACEMENT: 0 = off, "force" = always move (exercises the relocation)
+    const char* pe = getenv("HLBM_PLACEMENT");
+    const bool force = pe && std::strcmp(pe, "force") == 0;
+    if (!(pe && !force && atoi(pe) == 0)) {
+      const int rc = tune_placement(h, s, e0, e1, reps, 3, force, &moved);
+      if (rc) return done(rc);
+    }
+  }
   const TuneKey key{h->nx, h->ny, h->nz, h->d.storage, h->d.collision, h->d.pattern, h->plan.prog_planes,
                     h->any_wall && h->plan.wall_bb ? 1 : 0, h->device};
   {
@@ -950,38 +1006,17 @@ int hlbm_autotune(hlbm_domain* h, int32_t reps, void* stream, hlbm_tune_info* ou
     if (it != tune_cache.end()) {
       const TuneVal& v = it->second;
       hlbm_set_launch(h, v.bx, v.by, v.npt, v.remap, v.variant);
-      if (out) *out = hlbm_tune_info{v.bx, v.by, v.npt, v.remap, 0, v.variant, 0, 0, v.ms_best, v.ms_default};
-      return HLBM_OK;
+      if (out) *out = hlbm_tune_info{v.bx, v.by, v.npt, v.remap, 0, v.variant, moved, 0, v.ms_best, v.ms_default};
+      return done(HLBM_OK);
     }
   }
   void* scratch = nullptr;
   if (h->d.pattern == HLBM_PATTERN_AA) {
     const size_t fbytes = (size_t)h->stride * Q * h->esize;
-    if (hipMalloc(&scratch, fbytes) != hipSuccess) return fail(HLBM_ERR_NOMEM, "autotune scratch populations");
+    if (hipMalloc(&scratch, fbytes) != hipSuccess) return done(fail(HLBM_ERR_NOMEM, "autotune scratch populations"));
     if (hipMemcpyAsync(scratch, h->buf[h->cur], fbytes, hipMemcpyDeviceToDevice, s) != hipSuccess) {
       (void)hipFree(scratch);
-      return fail(HLBM_ERR_HIP, "autotune scratch copy");
-    }
-  }
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-    if (e0) (void)hipEventDestroy(e0);
-    if (scratch) (void)hipFree(scratch);
-    return fail(HLBM_ERR_HIP, "autotune events");
-  }
-  int moved = 0;
-  {
-    // HLBM_PLACEMENT: 0 = off, "force" = always move (exercises the relocation)
-    const char* pe = getenv("HLBM_PLACEMENT");
-    const bool force = pe && std::strcmp(pe, "force") == 0;
-    if (!(pe && !force && atoi(pe) == 0)) {
-      const int rc = tune_placement(h, s, e0, e1, reps, 3, force, &moved);
-      if (rc) {
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        if (scratch) (void)hipFree(scratch);
-        return rc;
-      }
+      return done(fail(HLBM_ERR_HIP, "autotune scratch copy"));
     }
   }
   const dim3 keep_block = h->block;

@@ -303,10 +303,10 @@ double hlbm_clock(void);
  * only the buffer the next step overwrites; AA domains work on a temporary
  * copy), keeps the fastest for this domain and caches it per (dims, storage,
  * collision, pattern, boundary program, device) so later domains of the same
- * shape skip the search (out->candidates == 0).  Before the search, an A/B
- * domain of 256 MiB of populations or more checks its placement: the step speed
+ * shape skip the search (out->candidates == 0).  First, a domain with 256 MiB
+ * of populations or more checks its placement: the step speed
  * depends on where the population buffers land in HBM (up to ~8 %, fixed per
- * allocation), so up to three fresh buffer pairs are timed and the state moves
+ * allocation), so up to three fresh buffers (pairs for A/B) are timed and the state moves
  * to a faster one (out->relocated; needs twice the population memory free for
  * the duration; HLBM_PLACEMENT=0 turns it off).  Results never depend on the
  * shape or the placement: every node runs the same arithmetic. */

@@ -71,21 +71,23 @@ def test_launch_shapes_with_boundary_program_planes(shape, pattern):
     b.dom.close()
 
 
-def test_placement_relocation_keeps_the_state(monkeypatch):
-    # 128^3 FP64 A/B: 319 MB of populations per buffer, so the autotuner checks
-    # the placement; forced, it moves the state to three fresh buffer pairs in turn
+@pytest.mark.parametrize("pattern,steps_before", [("ab", 4), ("aa", 4), ("aa", 3)])
+def test_placement_relocation_keeps_the_state(monkeypatch, pattern, steps_before):
+    # 128^3 FP64: 319 MB of populations per buffer, so the autotuner checks the
+    # placement; forced, it moves the state to three fresh buffers in turn (AA:
+    # after an even and an odd step count, the two layouts of the lattice)
     d = _couette_deck(128)
     f0 = seeded_equilibrium(d.nodes, 9)
-    ref = Simulation(d, storage="f64", init_f=f0)
+    ref = Simulation(d, storage="f64", init_f=f0, pattern=pattern)
     ref.run(9)
-    sim = Simulation(d, storage="f64", init_f=f0)
-    sim.run(4)
+    sim = Simulation(d, storage="f64", init_f=f0, pattern=pattern)
+    sim.run(steps_before)
     monkeypatch.setenv("HLBM_PLACEMENT", "force")
     info = sim.dom.autotune(reps=1)
     monkeypatch.delenv("HLBM_PLACEMENT")
     say("placement", info)
     assert info["relocated"] == 3
-    sim.run(5)
+    sim.run(9 - steps_before)
     assert np.array_equal(sim.dom.get_f(), ref.dom.get_f())
     rho, u = sim.dom.get_macro()
     rho_r, u_r = ref.dom.get_macro()
