@@ -292,31 +292,18 @@ __device__ __forceinline__ void bgk_f64(double* f, double rho, double ux, double
 
 // BGK on shifted populations g = f - w, FP32.
 __device__ __forceinline__ void bgk_shift(float* g, float dr, float ux, float uy, float uz, const StepParams& p) {
-  // g_i <- (1 - omega) g_i + omega geq_i, geq_i = w (dr + rho (3 eu + 4.5 eu^2 - 1.5 u^2)).
-  // Opposite directions share eu up to the sign: per pair (i, opp(i) = i + 1)
-  // the even part omega w (dr + rho (4.5 eu^2 - 1.5 u^2)) and the odd part
-  // 3 omega w rho eu are formed once.
   const float rho = 1.0f + dr;
-  const float omega = p.omega_f, omc = 1.0f - omega;
+  const float omega = p.omega_f;
   const float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
-  const float ow[3] = {omega * WF(0), omega * WF(1), omega * WF(7)};
-  const float ow3r[3] = {0.0f, 3.0f * ow[1] * rho, 3.0f * ow[2] * rho};
-  g[0] = omc * g[0] + ow[0] * (dr - rho * usq15);
-  static_for<1, Q>([&](auto ic) {
+  const bool force = p.has_force;
+  static_for([&](auto ic) {
     constexpr int i = decltype(ic)::value;
-    if constexpr (i & 1) {
-      constexpr int k = i < 7 ? 1 : 2;
-      const float eu = cdot<i>(ux, uy, uz);
-      const float even = ow[k] * (dr + rho * (4.5f * eu * eu - usq15));
-      const float odd = ow3r[k] * eu;
-      g[i] = omc * g[i] + (even + odd);
-      g[i + 1] = omc * g[i + 1] + (even - odd);
-    }
+    const float w = WF(i);
+    const float eu = cdot<i>(ux, uy, uz);
+    const float geq = w * (dr + rho * (eu * (3.0f + 4.5f * eu) - usq15));
+    g[i] = g[i] + omega * (geq - g[i]);
+    if (force) g[i] += rho * w * p.ef3f[i];
   });
-  if (p.has_force) {
-#pragma unroll
-    for (int i = 1; i < Q; i++) g[i] += rho * WF(i) * p.ef3f[i];
-  }
 }
 
 // ---------------------------------------------------------------------------
