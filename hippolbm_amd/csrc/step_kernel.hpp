@@ -758,35 +758,31 @@ __device__ __forceinline__ void apply_program(const StepParams& p, int x, int y,
             } else {
               // outflow (no counterpart in the reference): the populations entering
               // from outside take their post-stream values at the first interior node
-              static_for<0, 6>([&](auto pc) {
-                constexpr int pl = decltype(pc)::value;
-                constexpr int dim = pl < 2 ? 2 : (pl < 4 ? 1 : 0);
-                constexpr int inward = (pl & 1) ? -1 : 1;
-                if (e.plane == pl) {
-                  if (p.oflow) {
-                    // in-place AA step: the values were captured before any node was updated
-                    const C* cap = (const C*)p.oflow + (size_t)e.U[0] * 5 * (size_t)p.oflow_n +
-                                   plane_slot(pl, x, y, z, nx, ny);
-                    int j = 0;
-                    static_for<1, Q>([&](auto ic) {
-                      constexpr int i = decltype(ic)::value;
-                      constexpr int cd = dim == 0 ? CX(i) : (dim == 1 ? CY(i) : CZ(i));
-                      if constexpr (cd == inward) f[i] = cap[(size_t)(j++) * p.oflow_n];
-                    });
-                  } else {
-                    const int ax = x + (dim == 0 ? inward : 0), ay = y + (dim == 1 ? inward : 0),
-                              az = z + (dim == 2 ? inward : 0);
-                    const uint32_t aidx =
-                        ((uint32_t)az * (uint32_t)ny + (uint32_t)ay) * (uint32_t)nx + (uint32_t)ax;
-                    const bool awall = p.flags[aidx] != 0;
-                    static_for<1, Q>([&](auto ic) {
-                      constexpr int i = decltype(ic)::value;
-                      constexpr int cd = dim == 0 ? CX(i) : (dim == 1 ? CY(i) : CZ(i));
-                      if constexpr (cd == inward) f[i] = pull_one<S, MODE, i>(p, ax, ay, az, aidx, awall);
-                    });
-                  }
-                }
-              });
+              // (one copy of the code for the six planes keeps the register count down)
+              const int pl = e.plane;
+              const int dim = pl < 2 ? 2 : (pl < 4 ? 1 : 0);
+              const int inward = (pl & 1) ? -1 : 1;
+              if (p.oflow) {
+                // in-place AA step: the values were captured before any node was updated
+                const C* cap =
+                    (const C*)p.oflow + (size_t)e.U[0] * 5 * (size_t)p.oflow_n + plane_slot(pl, x, y, z, nx, ny);
+                int j = 0;
+                static_for<1, Q>([&](auto ic) {
+                  constexpr int i = decltype(ic)::value;
+                  const int cd = dim == 0 ? CX(i) : (dim == 1 ? CY(i) : CZ(i));
+                  if (cd == inward) f[i] = cap[(size_t)(j++) * p.oflow_n];
+                });
+              } else {
+                const int ax = x + (dim == 0 ? inward : 0), ay = y + (dim == 1 ? inward : 0),
+                          az = z + (dim == 2 ? inward : 0);
+                const uint32_t aidx = ((uint32_t)az * (uint32_t)ny + (uint32_t)ay) * (uint32_t)nx + (uint32_t)ax;
+                const bool awall = p.flags[aidx] != 0;
+                static_for<1, Q>([&](auto ic) {
+                  constexpr int i = decltype(ic)::value;
+                  const int cd = dim == 0 ? CX(i) : (dim == 1 ? CY(i) : CZ(i));
+                  if (cd == inward) f[i] = pull_one<S, MODE, i>(p, ax, ay, az, aidx, awall);
+                });
+              }
             }
           }
         }
