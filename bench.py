@@ -320,7 +320,7 @@ def main():
     ap.add_argument("--io", action="store_true", help="run only the output-operator measurements")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the domain-decomposed multi-GPU path is a design note, not code)", file=sys.stderr)
         return 2
     if args.copy:

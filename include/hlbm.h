@@ -4,9 +4,9 @@
  * This is the drop-in boundary for hippoLBM's hot path.  In the reference each
  * hot-path step is an Onika operator (C++ class + ADD_SLOT + execute()) that
  * mutates LBMFields<19> in place; this library replaces the bodies of those
- * operators.  A host-side operator layer (hippolbm_amd/csrc/operators.cpp, or
- * any Onika plugin) keeps the reference's operator names
- * and slots and calls the entry points below.
+ * operators.  A host-side operator layer (hippolbm_amd/host/operators.cpp, or
+ * an Onika plugin) keeps the reference's operator names and slots and calls
+ * the entry points below.
  *
  * Conventions
  *   - every function returns HLBM_OK (0) or a negative status; no exception

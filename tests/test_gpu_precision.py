@@ -1,6 +1,6 @@
 """Reduced-precision forms (FP32 compute on shifted f - w stored as FP32, FP16,
 BF16) against the FP64 engine on the same inputs.  Tolerances are the error
-bounds this build states per storage type."""
+bounds stated per storage type."""
 import numpy as np
 import pytest
 
