@@ -160,9 +160,13 @@ struct StoreOps<_Float16> {
     __builtin_nontemporal_store((_Float16)(v * SCALE), p + i);
   }
   static __device__ __forceinline__ void st_sr(_Float16* p, size_t i, C v, uint32_t r) {
-    // FP16 keeps 10 of FP32's 23 mantissa bits: dither the 13 dropped ones
-    const uint32_t u = (__float_as_uint(v * SCALE) + (r >> 19)) & 0xFFFFE000u;
-    __builtin_nontemporal_store((_Float16)__uint_as_float(u), p + i);
+    // FP16 keeps 10 of FP32's 23 mantissa bits: the 13 dropped ones are dithered
+    // with the top 13 bits of r, i.e. (bits(v) + (r >> 19)) & ~0x1FFF, then
+    // converted; gfx950's v_cvt_sr_f16_f32 does exactly this in one instruction
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const h2 z = {(_Float16)0.0f, (_Float16)0.0f};
+    const h2 h = __builtin_amdgcn_cvt_sr_f16_f32(z, v * SCALE, r, false);
+    __builtin_nontemporal_store(h[0], p + i);
   }
 };
 template <>
@@ -179,8 +183,12 @@ struct StoreOps<bf16s> {
     __builtin_nontemporal_store((uint16_t)(u >> 16), (uint16_t*)(p + i));
   }
   static __device__ __forceinline__ void st_sr(bf16s* p, size_t i, C v, uint32_t r) {
-    // dither the 16 dropped bits
-    __builtin_nontemporal_store((uint16_t)((__float_as_uint(v) + (r >> 16)) >> 16), (uint16_t*)(p + i));
+    // the 16 dropped bits dithered with the top 16 bits of r, (bits(v) + (r >> 16)) >> 16:
+    // gfx950's v_cvt_sr_bf16_f32
+    typedef short s2 __attribute__((ext_vector_type(2)));
+    const s2 z = {0, 0};
+    const s2 b = __builtin_amdgcn_cvt_sr_bf16_f32(z, v, r, false);
+    __builtin_nontemporal_store((uint16_t)b[0], (uint16_t*)(p + i));
   }
 };
 
