@@ -18,7 +18,10 @@ from hippolbm_amd import _lib
 from hippolbm_amd import deck as dk
 from hippolbm_amd.engine import Domain, Simulation
 
-from conftest import DECKS
+from conftest import DECKS, GOLDEN
+
+sys.path.insert(0, GOLDEN)
+from cases import seeded_equilibrium  # noqa: E402,F401  (also used by the other GPU tests)
 
 pytestmark = pytest.mark.gpu
 
@@ -90,21 +93,6 @@ post_stream_bcs:
   - post_bounce_back
 """
     return dk.loads_msp(txt, "couette")
-
-
-def seeded_equilibrium(nodes, seed, amp=1e-3):
-    """rho = 1 + U(-amp, amp) (splitmix64 -> 53-bit uniform), u = 0, f = f_eq."""
-    n = nodes[0] * nodes[1] * nodes[2]
-    with np.errstate(over="ignore"):
-        # state_k = seed + (k+1)*gamma (mod 2^64), then the splitmix64 finalizer
-        z = np.uint64(seed) + (np.arange(1, n + 1, dtype=np.uint64) * np.uint64(0x9E3779B97F4A7C15))
-        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
-        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
-        z = z ^ (z >> np.uint64(31))
-    out = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
-    rho = 1.0 + (2.0 * out - 1.0) * amp
-    w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
-    return (w[:, None] * rho[None, :]).reshape(19, nodes[2], nodes[1], nodes[0])
 
 
 def test_seeded_couette_bitwise():
