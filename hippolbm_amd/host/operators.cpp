@@ -792,7 +792,9 @@ void Simulation::compute_loop() {
     // end_iteration (config_compute_loop.msp:8-16): dump_paraview_if_triggered
     // (update_ghost + macro_variables + write_paraview), print_log_if_triggered
     // (simulation_state + log), print_analysis_if_triggered
-    if (s.paraview_freq > 0 && s.timestep % s.paraview_freq == 0 && s.io) {
+    if (s.paraview_freq > 0 && s.timestep % s.paraview_freq == 0) {
+      // the dump's macro_variables also changes what the log below reports, so it
+      // runs with --no-output too
       s.flush();
       check(hlbm_macro(s.dom, nullptr, nullptr, HLBM_LAYOUT_NODES), "macro_variables");
       paraview->execute(s);
