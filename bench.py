@@ -97,10 +97,11 @@ def porous_channel(steps=100, warmup=10, storage="f64"):
             "ms_per_step": ms_step, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS, "launch": tuned}
 
 
-def batched_cavities(n_dom=2048, chunk=200, seed=4, storage="f16"):
+def batched_cavities(n_dom=2048, chunk=200, seed=4, storage="f16", use_graph=True):
     """2048 independent lid-driven cavities, edges ~U{16..96} per axis, lid speed
     ~U[0.01,0.1], tau ~U[0.52,1.0], BGK, one persistent launch per step, chunks
-    of `chunk` steps replayed from a captured HIP graph."""
+    of `chunk` steps replayed from a captured HIP graph (use_graph=False: the
+    same launches issued one by one on the stream)."""
     import numpy as np
     from hippolbm_amd.batch import Batch, Solve
     rng = np.random.default_rng(seed)
@@ -111,7 +112,7 @@ def batched_cavities(n_dom=2048, chunk=200, seed=4, storage="f16"):
                             cavity_zl=(float(rng.uniform(0.01, 0.1)), 0.0, 0.0)))
     nodes_total = sum(s.node_count() for s in solves)
     b = Batch(storage=storage, budget_bytes=nodes_total * (38 * BYTES[storage] + 1) + (2 << 30),
-              chunk_steps=chunk, use_graph=True)
+              chunk_steps=chunk, use_graph=use_graph)
     for s in solves:
         b.submit(s)
     b.run()  # admission + first chunk (warm-up)
@@ -126,18 +127,19 @@ def batched_cavities(n_dom=2048, chunk=200, seed=4, storage="f16"):
             "ms_per_step": ms / chunk, "frac": mlups * 1e6 * bpn / 1e9 / HBM_PEAK_GBS}
 
 
-def served_sweep(rates=(4.0, 16.0, 64.0, 0.0), n=32, seed=5, storages=("f32", "f16"), chunk=200, budget_gb=96):
+def served_sweep(rates=(4.0, 16.0, 64.0, 0.0), n=32, seed=5, storages=("f32", "f16"), chunk=200, budget_gb=96,
+                 use_graph=True):
     """served_stream at several arrival rates (0 = every request at t = 0), per storage."""
-    return {st: [served_stream(n, r, seed, st, chunk, budget_gb) for r in rates] for st in storages}
+    return {st: [served_stream(n, r, seed, st, chunk, budget_gb, use_graph) for r in rates] for st in storages}
 
 
-def served_stream(n=32, rate_hz=4.0, seed=5, storage="f16", chunk=200, budget_gb=96):
+def served_stream(n=32, rate_hz=4.0, seed=5, storage="f16", chunk=200, budget_gb=96, use_graph=True):
     """Poisson arrivals of cavity / channel / porous solves (sizes log-uniform
     32^3..256^3, 500..5000 steps, BGK or MRT), HBM-budget admission, continuous
     batching; time-to-result percentiles and aggregate MLUPS."""
     from hippolbm_amd.batch import Batch, poisson_requests, serve
     reqs = poisson_requests(n, rate_hz, seed)
-    b = Batch(storage=storage, budget_bytes=budget_gb << 30, chunk_steps=chunk, use_graph=True)
+    b = Batch(storage=storage, budget_bytes=budget_gb << 30, chunk_steps=chunk, use_graph=use_graph)
     r = serve(b, reqs)
     b.close()
     bpn = 2 * 19 * BYTES[storage] + 1
@@ -313,7 +315,11 @@ def main():
     ap.add_argument("--only", default=None, help="run a single storage type (profiling)")
     ap.add_argument("--copy", action="store_true", help="run only the streaming-copy reference kernel (profiling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-cavity and served-stream legs")
-    ap.add_argument("--batch-only", action="store_true", help="run only the batched and served legs")
+    ap.add_argument("--batch-only", nargs="?", const="both", choices=("both", "batched", "served"), default=None,
+                    help="run only the batched and/or served legs")
+    ap.add_argument("--batch-launch", choices=("graph", "stream"), default="graph",
+                    help="batched and served legs: replay each chunk from a HIP graph, or launch its steps "
+                         "one by one on the stream (per-dispatch profiling)")
     ap.add_argument("--precision", type=int, default=0, metavar="STEPS",
                     help="run only the precision sweep (Couette 512^3, Poiseuille 512x256x256) for STEPS steps")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
@@ -338,7 +344,13 @@ def main():
         print(json.dumps({"storage": args.only, **r}), flush=True)
         return 0
     if args.batch_only:
-        print(json.dumps({"batched": batched_cavities(), "served": served_sweep()}), flush=True)
+        g = args.batch_launch == "graph"
+        out = {"batch_launch": args.batch_launch}
+        if args.batch_only in ("both", "batched"):
+            out["batched"] = batched_cavities(use_graph=g)
+        if args.batch_only in ("both", "served"):
+            out["served"] = served_sweep(use_graph=g)
+        print(json.dumps(out), flush=True)
         return 0
     n = args.size
     per = {}

@@ -1431,13 +1431,49 @@ __device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, c
   return p;
 }
 
+// One node of a batch tile.  WALLS: the solve has obstacles, so the node's
+// bounce-back mask is read; a solve without obstacles (closed cavities, empty
+// channels) compiles the mask out, and only its faces bounce back.
+template <class S, int COLL, bool WALLS>
+__device__ __forceinline__ void batch_node(const BatchParams& bp, const BatchDomain& d, const StepParams& p,
+                                           uint32_t n) {
+  using C = typename StoreOps<S>::C;
+  // floor((n + 1/2) / d) from the rounded reciprocal is exact: the quotient's
+  // distance to an integer, >= 1/(2d), exceeds the relative rounding error 2^-52
+  const uint32_t nxy = (uint32_t)p.nx * (uint32_t)p.ny;
+  const uint32_t zu = (uint32_t)(((double)n + 0.5) * d.inv_nxy);
+  const uint32_t rem = n - zu * nxy;
+  const uint32_t yu = (uint32_t)(((double)rem + 0.5) * d.inv_nx);
+  const int z = (int)zu, y = (int)yu, x = (int)(rem - yu * (uint32_t)p.nx);
+  uint32_t m = 0u;
+  if constexpr (WALLS) m = d.any_wall ? bp.bbm[d.flag_off + n] : 0u;
+  const bool wall = WALLS && (m & 1u) != 0u;
+  const uint32_t wbits = WALLS ? (d.wall_bb ? m : (m & 1u)) : 0u;
+  const uint32_t bo = n * (uint32_t)sizeof(S);
+  const NodeNbrs a = node_nbrs<S>(p, x, y, z);
+  C f[Q];
+  gather_select<S, MODE_FUSED>(p, bo, a, wbits, f);
+  collide_node<S, COLL, MODE_FUSED>(p, n, wall, f);
+  store_bulk<S, MODE_FUSED>(p, bo, a, wbits, f);
+}
+
 // One launch per collision model and time step: the BGK launch walks the
 // BGK tiles, the MRT launch the MRT tiles, so each kernel keeps only its own
 // collision in registers.  Tiles are 256 consecutive nodes of one domain, so
 // the domain's parameters are workgroup-uniform.
+//
+// The BGK kernels request the next tile's entry while the current tile's
+// domain parameters load (one scalar-load latency per tile instead of two).
+// With 16-bit storage the BGK kernel also compiles a second node path for
+// solves without obstacles (no mask read) and asks for 8 waves per SIMD, which
+// the register allocator meets without spilling vector registers; the wider
+// kernels and the MRT kernels keep one path, since the second one costs them
+// a wave per SIMD.
 template <class S, int COLL>
-__global__ void __launch_bounds__(256) batch_kernel(const BatchParams bp) {
-  using C = typename StoreOps<S>::C;
+__global__ void __launch_bounds__(256, (sizeof(S) == 2 && COLL == COLL_BGK) ? 8 : 1)
+    batch_kernel(const BatchParams bp) {
+  constexpr bool PREFETCH = COLL == COLL_BGK;
+  constexpr bool TWO_PATHS = sizeof(S) == 2 && COLL == COLL_BGK;
   uint32_t t0 = COLL == COLL_BGK ? 0u : bp.ntiles[0];
   uint32_t t1 = COLL == COLL_BGK ? bp.ntiles[0] : bp.ntiles[1];
   uint32_t first = blockIdx.x, stride = gridDim.x;
@@ -1451,30 +1487,27 @@ __global__ void __launch_bounds__(256) batch_kernel(const BatchParams bp) {
     first = blockIdx.x >> 3;
     stride = gridDim.x >> 3;
   }
-  for (uint32_t t = t0 + first; t < t1; t += stride) {
-    const BatchTile tl = bp.tiles[t];
+  uint32_t t = t0 + first;
+  BatchTile next = BatchTile{0u, 0u};
+  if (PREFETCH && t < t1) next = bp.tiles[t];
+  for (; t < t1; t += stride) {
+    BatchTile tl;
+    if constexpr (PREFETCH) {
+      tl = next;
+      if (t + stride < t1) next = bp.tiles[t + stride];
+    } else {
+      tl = bp.tiles[t];
+    }
     const BatchDomain& d = bp.doms[tl.dom];
     if (bp.step >= d.active_steps) continue;
     const int parity = (d.parity + bp.step) & 1;
-    StepParams p = batch_step_params<S>(bp, d, parity);
+    const StepParams p = batch_step_params<S>(bp, d, parity);
     const uint32_t n = tl.first + threadIdx.x;
     if (n >= p.N) continue;
-    // floor((n + 1/2) / d) from the rounded reciprocal is exact: the quotient's
-    // distance to an integer, >= 1/(2d), exceeds the relative rounding error 2^-52
-    const uint32_t nxy = (uint32_t)p.nx * (uint32_t)p.ny;
-    const uint32_t zu = (uint32_t)(((double)n + 0.5) * d.inv_nxy);
-    const uint32_t rem = n - zu * nxy;
-    const uint32_t yu = (uint32_t)(((double)rem + 0.5) * d.inv_nx);
-    const int z = (int)zu, y = (int)yu, x = (int)(rem - yu * (uint32_t)p.nx);
-    const uint32_t m = d.any_wall ? bp.bbm[d.flag_off + n] : 0u;
-    const bool wall = (m & 1u) != 0u;
-    const uint32_t wbits = d.wall_bb ? m : (m & 1u);
-    const uint32_t bo = n * (uint32_t)sizeof(S);
-    const NodeNbrs a = node_nbrs<S>(p, x, y, z);
-    C f[Q];
-    gather_select<S, MODE_FUSED>(p, bo, a, wbits, f);
-    collide_node<S, COLL, MODE_FUSED>(p, n, wall, f);
-    store_bulk<S, MODE_FUSED>(p, bo, a, wbits, f);
+    if (TWO_PATHS && !d.any_wall)
+      batch_node<S, COLL, false>(bp, d, p, n);
+    else
+      batch_node<S, COLL, true>(bp, d, p, n);
   }
 }
 
