@@ -155,7 +155,11 @@ struct StoreOps<_Float16> {
   static constexpr bool shifted = true;
   static constexpr bool narrow16 = true;
   static constexpr float SCALE = 8192.0f, INV_SCALE = 1.0f / 8192.0f;  // 2^13, 2^-13
-  static __device__ __forceinline__ C ld(const _Float16* p, size_t i) { return (float)p[i] * INV_SCALE; }
+  // fma(h, 2^-13, +0) is h * 2^-13 (exact) except that -0 loads as +0, and it
+  // is one v_fma_mix_f32 (conversion and scaling together) instead of two
+  static __device__ __forceinline__ C ld(const _Float16* p, size_t i) {
+    return __builtin_fmaf((float)p[i], INV_SCALE, 0.0f);
+  }
   static __device__ __forceinline__ void st(_Float16* p, size_t i, C v) {
     __builtin_nontemporal_store((_Float16)(v * SCALE), p + i);
   }
