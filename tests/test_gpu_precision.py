@@ -92,6 +92,38 @@ def test_stochastic_rounding_keeps_long_runs_accurate():
     assert err["nearest"] > 3 * err["stochastic"]
 
 
+# max |u_x - closed form| / u_max of steady plane Poiseuille per storage.  FP64
+# carries only the discretisation error of half-way bounce-back under BGK at
+# tau = 1 (1.15e-3: the oracle's figure on the same deck); the reduced forms add
+# their storage error on top.
+POIS_TOL = {"f64": 1.5e-3, "f32": 1.5e-3, "f16": 4e-3, "bf16": 3e-2}
+
+
+@pytest.mark.parametrize("storage", ["f64", "f32", "f16", "bf16"])
+def test_poiseuille_steady_profile_per_storage(storage):
+    # periodic x/y, bounce-back plates half a spacing outside z = 0 and z = 16
+    # (H = 17), body force F along x; 8000 steps = 4.6 diffusion times H^2/nu:
+    # u_x(z) = F / (2 nu) (z + 1/2) (H - z - 1/2), u_y = u_z = 0
+    from hippolbm_amd import workloads
+    nz, F, tau = 17, 1e-5, 1.0
+    d = workloads.poiseuille((4, 3, nz), F=F, tau=tau)
+    s = Simulation(d, storage=storage)
+    s.run(8000)
+    _, u = s.dom.macro()
+    s.dom.close()
+    nu = (tau - 0.5) / 3
+    z = np.arange(nz) + 0.5
+    ana = F / (2 * nu) * z * (nz - z)
+    ux = u[0].mean(axis=(1, 2))
+    err = np.abs(ux - ana).max() / ana.max()
+    spread = np.abs(u[0] - ux[:, None, None]).max() / ana.max()
+    cross = max(np.abs(u[1]).max(), np.abs(u[2]).max()) / ana.max()
+    print(f"{storage}: max|ux - closed form|/u_max = {err:.3e}, in-plane spread {spread:.1e}, "
+          f"|uy|,|uz| {cross:.1e}", flush=True)
+    assert err < POIS_TOL[storage]
+    assert spread < POIS_TOL[storage] and cross < POIS_TOL[storage]
+
+
 def test_stochastic_rounding_is_deterministic():
     d = deck(24)
     a = Simulation(d, storage="f16")
