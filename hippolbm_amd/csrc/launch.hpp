@@ -32,6 +32,11 @@ inline void set_fp32_constants(StepParams& p) {
   p.omega_f = (float)(1.0 / p.tau);
 }
 
+// Program-plane launches use small workgroups: a few planes of nodes are too
+// few 256-thread workgroups to reach every CU, and each lane's accesses are
+// latency-bound (on an x-normal plane every lane touches its own cache lines).
+constexpr unsigned BOUNDARY_BLOCK = 64;
+
 inline unsigned boundary_blocks(const StepParams& p) {
   uint64_t n = 0;
   for (int pl = 0; pl < 6; pl++)
@@ -39,7 +44,7 @@ inline unsigned boundary_blocks(const StepParams& p) {
       const int dim = plane_dim(pl);
       n += dim == 0 ? (uint64_t)p.ny * p.nz : (dim == 1 ? (uint64_t)p.nx * p.nz : (uint64_t)p.nx * p.ny);
     }
-  return (unsigned)((n + 255) / 256);
+  return (unsigned)((n + BOUNDARY_BLOCK - 1) / BOUNDARY_BLOCK);
 }
 
 // FP64 storage, FP64 compute, no contraction (step_f64.hip)

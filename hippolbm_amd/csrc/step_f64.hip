@@ -19,7 +19,7 @@ static hipError_t go(dim3 grid, dim3 block, const StepParams& p, hipStream_t s) 
   // slots (AA), so either kernel may run first
   const bool planes = MODE != MODE_COLLIDE && p.prog_planes;
   if (planes && p.boundary_first) {
-    hipLaunchKernelGGL((boundary_kernel<double, COLL, MODE>), dim3(boundary_blocks(p)), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((boundary_kernel<double, COLL, MODE>), dim3(boundary_blocks(p)), dim3(BOUNDARY_BLOCK), 0, s, p);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -28,7 +28,7 @@ static hipError_t go(dim3 grid, dim3 block, const StepParams& p, hipStream_t s) 
     e = hipGetLastError();
   }
   if (e != hipSuccess || !planes || p.boundary_first) return e;
-  hipLaunchKernelGGL((boundary_kernel<double, COLL, MODE>), dim3(boundary_blocks(p)), dim3(256), 0, s, p);
+  hipLaunchKernelGGL((boundary_kernel<double, COLL, MODE>), dim3(boundary_blocks(p)), dim3(BOUNDARY_BLOCK), 0, s, p);
   return hipGetLastError();
 }
 
