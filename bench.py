@@ -324,6 +324,7 @@ def main():
                     help="run only the precision sweep (Couette 512^3, Poiseuille 512x256x256) for STEPS steps")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
     ap.add_argument("--io", action="store_true", help="run only the output-operator measurements")
+    ap.add_argument("--porous", action="store_true", help="run only the porous-channel leg (profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -335,6 +336,9 @@ def main():
         return 0
     if args.io:
         print(json.dumps({"output_operators": output_operators(args.size)}), flush=True)
+        return 0
+    if args.porous:
+        print(json.dumps({"porous_channel": porous_channel(args.steps, args.warmup)}), flush=True)
         return 0
     if args.precision:
         print(json.dumps({"precision_sweep": precision_sweep(args.precision)}), flush=True)
