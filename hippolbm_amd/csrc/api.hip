@@ -913,6 +913,9 @@ static int tune_placement(hlbm_domain* h, hipStream_t s, hipEvent_t e0, hipEvent
       return fail(HLBM_ERR_HIP, "autotune: placement copy");
     }
   }
+  // rejected candidates stay allocated until the search ends, so that each
+  // try is handed a region of its own rather than the one just freed
+  std::vector<void*> rejected;
   double best = 0.0;
   int rc = time_shape(h, aa ? h->buf[h->cur] : nullptr, reps, s, e0, e1, &best);
   if (rc == HLBM_OK && aa && (!copy(h->buf[h->cur], save) || hipStreamSynchronize(s) != hipSuccess))
@@ -962,9 +965,10 @@ static int tune_placement(hlbm_domain* h, hipStream_t s, hipEvent_t e0, hipEvent
       }
     } else {
       for (void* q : nb)
-        if (q) (void)hipFree(q);
+        if (q) rejected.push_back(q);
     }
   }
+  for (void* q : rejected) (void)hipFree(q);
   if (save) (void)hipFree(save);
   return rc;
 }

@@ -307,8 +307,10 @@ double hlbm_clock(void);
  * of populations or more checks its placement: the step speed
  * depends on where the population buffers land in HBM (up to ~8 %, fixed per
  * allocation), so up to three fresh buffers (pairs for A/B) are timed and the state moves
- * to a faster one (out->relocated; needs twice the population memory free for
- * the duration; HLBM_PLACEMENT=0 turns it off).  Results never depend on the
+ * to a faster one (out->relocated; rejected buffers stay allocated until the
+ * search ends so that each try lands elsewhere, so it uses up to four times the
+ * population memory while free memory allows, keeping 4 GiB free;
+ * HLBM_PLACEMENT=0 turns it off).  Results never depend on the
  * shape or the placement: every node runs the same arithmetic. */
 typedef struct {
   int32_t block_x, block_y, npt, xcd_remap; /* chosen shape */
