@@ -84,7 +84,7 @@ def porous_channel(steps=100, warmup=10, storage="f64"):
     from hippolbm_amd.engine import Simulation
     d = workloads.porous_channel()
     sim = Simulation(d, storage=storage)
-    tuned = sim.dom.autotune(reps=20)  # a step is ~0.33 ms: 20 steps per candidate keeps the timing above noise
+    tuned = sim.dom.autotune(reps=3)
     ms, _ = time_steps(sim.dom, steps, warmup)
     flags = sim.dom.get_flags()
     nodes = sim.dom.N

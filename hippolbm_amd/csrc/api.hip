@@ -1028,6 +1028,13 @@ int hlbm_autotune(hlbm_domain* h, int32_t reps, void* stream, hlbm_tune_info* ou
   static const int shapes[][2] = {{128, 2}, {64, 1}, {64, 2}, {64, 4}, {128, 1}, {256, 1}};
   TuneVal best{128, 2, 1, 1, 0, 1e300, 0.0};
   int rc = HLBM_OK, n = 0;
+  {
+    // short steps: time enough of them that each candidate's measurement spans
+    // about 5 ms (at most 500 steps), else launch jitter picks the shape
+    double ms = 0.0;
+    rc = time_shape(h, scratch, reps, s, e0, e1, &ms);
+    if (rc == HLBM_OK && ms > 0.0 && reps * ms < 5.0) reps = std::min(500, (int)std::ceil(5.0 / ms));
+  }
   for (int variant = 0; variant <= 1 && rc == HLBM_OK; variant++) {
     for (const auto& sh : shapes) {
       for (int npt = 1; npt <= (variant ? 1 : 2) && rc == HLBM_OK; npt++) {

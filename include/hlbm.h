@@ -303,7 +303,9 @@ double hlbm_clock(void);
  * only the buffer the next step overwrites; AA domains work on a temporary
  * copy), keeps the fastest for this domain and caches it per (dims, storage,
  * collision, pattern, boundary program, device) so later domains of the same
- * shape skip the search (out->candidates == 0).  First, a domain with 256 MiB
+ * shape skip the search (out->candidates == 0).  Each candidate is timed over
+ * `reps` steps, or over as many as take about 5 ms when `reps` steps of the
+ * default shape take less (at most 500).  First, a domain with 256 MiB
  * of populations or more checks its placement: the step speed
  * depends on where the population buffers land in HBM (up to ~8 %, fixed per
  * allocation), so up to three fresh buffers (pairs for A/B) are timed and the state moves
