@@ -35,7 +35,7 @@ SYMBOLS = (
     "hlbm_reduce_stats", "hlbm_ghost_update_ref", "hlbm_copy_bench", "hlbm_autotune", "hlbm_set_launch",
     "hlbm_set_collision", "hlbm_plane_profile", "hlbm_line_velocity", "hlbm_init_double_couette",
     "hlbm_batch_create", "hlbm_batch_destroy", "hlbm_batch_submit", "hlbm_batch_run", "hlbm_batch_status",
-    "hlbm_batch_times", "hlbm_batch_result", "hlbm_batch_info_get", "hlbm_clock",
+    "hlbm_batch_times", "hlbm_batch_result", "hlbm_batch_stats", "hlbm_batch_info_get", "hlbm_clock",
 )
 
 

@@ -5,7 +5,9 @@ run together: one persistent kernel launch per time step over all running
 solves, chunks of ``chunk_steps`` replayed from a captured HIP graph, solves
 admitted as memory frees up and retired (rho/u copied out) as they finish.
 ``serve`` drives a Poisson stream of requests through it and reports
-time-to-result percentiles beside the aggregate MLUPS.
+time-to-result percentiles beside the aggregate MLUPS.  A solve whose final
+state is not finite ends FAILED (its own status; the other solves are not
+affected).
 """
 from __future__ import annotations
 
@@ -32,14 +34,19 @@ class SolveDesc(ctypes.Structure):
         ("cavity_on", ctypes.c_int32 * 2), ("keep_populations", ctypes.c_int32), ("mrt_custom", ctypes.c_int32),
         ("tau", ctypes.c_double), ("fext", ctypes.c_double * 3), ("dx", ctypes.c_double),
         ("init_rho", ctypes.c_double), ("cavity_u", (ctypes.c_double * 3) * 2), ("mrt_rates", ctypes.c_double * 10),
-        ("obstacles", ctypes.POINTER(_lib.Obstacle)),
+        ("obstacles", ctypes.POINTER(_lib.Obstacle)), ("init_f", ctypes.POINTER(ctypes.c_double)),
     ]
+
+
+# enum hlbm_solve_state
+QUEUED, RUNNING, DONE, FAILED = 0, 1, 2, 3
 
 
 class BatchInfo(ctypes.Structure):
     _fields_ = [("queued", ctypes.c_int64), ("running", ctypes.c_int64), ("done", ctypes.c_int64),
                 ("chunks", ctypes.c_int64), ("bytes_in_use", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64),
-                ("node_updates", ctypes.c_double), ("gpu_ms", ctypes.c_double)]
+                ("node_updates", ctypes.c_double), ("gpu_ms", ctypes.c_double),
+                ("peak_bytes_in_use", ctypes.c_uint64)]
 
 
 _bound = False
@@ -60,6 +67,7 @@ def _bind():
         "hlbm_batch_status": ([P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32), I64P], ctypes.c_int),
         "hlbm_batch_times": ([P, ctypes.c_int64, DP, DP, DP], ctypes.c_int),
         "hlbm_batch_result": ([P, ctypes.c_int64, DP, DP, DP], ctypes.c_int),
+        "hlbm_batch_stats": ([P, ctypes.c_int64, DP], ctypes.c_int),
         "hlbm_batch_info_get": ([P, ctypes.POINTER(BatchInfo)], ctypes.c_int),
         "hlbm_clock": ([], ctypes.c_double),
     }
@@ -86,6 +94,7 @@ class Solve:
     dx: float = 1.0
     init_rho: float = 1.0
     keep_populations: bool = False
+    init_f: object = None   # optional FP64 initial populations, shape (19, nz, ny, nx)
 
     def node_count(self):
         return int(self.nodes[0]) * int(self.nodes[1]) * int(self.nodes[2])
@@ -114,7 +123,13 @@ class Solve:
             d.obstacles = ctypes.cast(keep, ctypes.POINTER(_lib.Obstacle))
             d.n_obstacles = len(obs)
         d.wall_bb = 1 if (self.wall_bb and obs) else 0
-        return d, keep
+        f0 = None
+        if self.init_f is not None:
+            f0 = np.ascontiguousarray(self.init_f, dtype=np.float64).reshape(-1)
+            if f0.size != 19 * self.node_count():
+                raise ValueError("init_f must hold 19 populations per node")
+            d.init_f = f0.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        return d, (keep, f0)
 
     def run_single(self, storage="f64"):
         """The same solve on the single-domain path (reference operator order)."""
@@ -133,6 +148,8 @@ class Solve:
         dom.init_obst()
         dom.set_distribution(self.init_rho)
         dom.add_obstacles(self.obstacles)
+        if self.init_f is not None:
+            dom.set_f(np.asarray(self.init_f, dtype=np.float64).reshape((19,) + tuple(self.nodes[::-1])))
         dom.step(self.steps)
         rho, u = dom.macro()
         f = dom.get_f() if self.keep_populations else None
@@ -184,6 +201,12 @@ class Batch:
         check("hlbm_batch_status", lib().hlbm_batch_status(self.h, sid, ctypes.byref(st), ctypes.byref(done)))
         return st.value, done.value
 
+    def stats(self, sid):
+        """simulation_state of a finished solve: sum rho, min |u|, max |u|, sum rho u (3), fluid nodes."""
+        out = np.zeros(7)
+        check("hlbm_batch_stats", lib().hlbm_batch_stats(self.h, sid, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out
+
     def times(self, sid):
         a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         check("hlbm_batch_times", lib().hlbm_batch_times(self.h, sid, ctypes.byref(a), ctypes.byref(b),
@@ -202,6 +225,11 @@ class Batch:
             u.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), fp))
         shp = (s.nodes[2], s.nodes[1], s.nodes[0])
         return rho.reshape(shp), u.reshape((3,) + shp), (f.reshape((19,) + shp) if f is not None else None)
+
+    def release(self, sid):
+        """Forget a finished solve without copying its fields."""
+        self.specs.pop(sid)
+        check("hlbm_batch_result", lib().hlbm_batch_result(self.h, sid, None, None, None))
 
     def info(self):
         i = BatchInfo()
@@ -260,6 +288,7 @@ def serve(batch, requests, keep_results=False, poll_sleep=0.0005):
     inflight = {}
     lat = []
     results = {}
+    failed = []
     node_updates = 0.0
     while pending or inflight:
         now = clock() - t0
@@ -273,17 +302,20 @@ def serve(batch, requests, keep_results=False, poll_sleep=0.0005):
         batch.run()
         for sid in list(inflight):
             st, _ = batch.state(sid)
-            if st == 2:
+            if st in (DONE, FAILED):
                 arr, spec = inflight.pop(sid)
                 _, _, t_done = batch.times(sid)
                 lat.append(t_done - t0 - arr)
                 node_updates += spec.node_count() * spec.steps
+                if st == FAILED:
+                    failed.append(sid)
                 r = batch.result(sid)
                 if keep_results:
-                    results[sid] = (spec, r)
+                    results[sid] = (spec, r, st)
     makespan = clock() - t0
     lat = np.array(lat)
     gpu_s = batch.info()["gpu_ms"] * 1e-3 - gpu0
     return {"n": len(lat), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
             "mean_s": float(lat.mean()), "makespan_s": makespan, "node_updates": node_updates,
-            "mlups": node_updates / makespan / 1e6, "gpu_busy": gpu_s / makespan, "results": results}
+            "mlups": node_updates / makespan / 1e6, "gpu_busy": gpu_s / makespan, "failed": failed,
+            "results": results}

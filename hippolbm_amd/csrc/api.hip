@@ -214,7 +214,8 @@ int hlbm_domain_create(const hlbm_domain_desc* desc, hlbm_domain** out) {
   if (desc->storage < HLBM_F64 || desc->storage > HLBM_BF16) return fail(HLBM_ERR_ARG, "bad storage");
   if (desc->collision != HLBM_BGK && desc->collision != HLBM_MRT) return fail(HLBM_ERR_ARG, "bad collision");
   if (desc->pattern != HLBM_PATTERN_AB && desc->pattern != HLBM_PATTERN_AA) return fail(HLBM_ERR_ARG, "bad pattern");
-  if (!(desc->tau > 0.0)) return fail(HLBM_ERR_ARG, "tau must be > 0");
+  // tau <= 1/2 means a zero or negative viscosity, nu = (tau - 1/2)/3: the update is unstable
+  if (!(desc->tau > 0.5)) return fail(HLBM_ERR_ARG, "tau must be > 0.5 (nu = (tau - 0.5)/3 > 0)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(HLBM_ERR_NODEVICE, "no HIP device visible");
 
@@ -295,7 +296,7 @@ int hlbm_domain_bytes(const hlbm_domain* h, uint64_t* bytes) {
 
 int hlbm_set_params(hlbm_domain* h, double tau, const double fext[3]) {
   if (!h || !fext) return fail(HLBM_ERR_ARG, "NULL argument");
-  if (!(tau > 0.0)) return fail(HLBM_ERR_ARG, "tau must be > 0");
+  if (!(tau > 0.5)) return fail(HLBM_ERR_ARG, "tau must be > 0.5 (nu = (tau - 0.5)/3 > 0)");
   h->d.tau = tau;
   for (int d = 0; d < 3; d++) h->d.fext[d] = fext[d];
   return HLBM_OK;

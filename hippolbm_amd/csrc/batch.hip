@@ -17,6 +17,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <deque>
@@ -96,8 +97,10 @@ struct Solve {
   double t_submit = 0, t_admit = 0, t_done = 0;
   bool retiring = false;      // retirement enqueued; DONE once `ev` has completed
   hipEvent_t ev = nullptr;
-  int res_arena = 0;          // arena whose slice holds rho (N) and u (3N), as doubles
+  int res_arena = 0;          // arena whose slice holds rho (N), u (3N) and the statistics (7), as doubles
   std::vector<double> rho, u, f;
+  std::vector<double> init_f; // initial populations (FP64 node layout), empty = init_rho * w
+  double stats[7] = {0, 0, 0, 0, 0, 0, 0};
 };
 
 uint64_t round_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
@@ -127,7 +130,8 @@ struct hlbm_batch {
   ObstacleDev* d_obs[2] = {nullptr, nullptr};
   size_t obs_cap = 0, obs_used[2] = {0, 0};
   uint64_t max_doms = 0, max_tiles = 0;
-  double* f_tmp = nullptr;  // one FP64 population plane (keep_populations)
+  double* f_tmp = nullptr;  // one FP64 population plane (keep_populations, initial populations)
+  double* d_part = nullptr; // simulation_state partial sums (retirement, one reduction at a time on `stream`)
   uint64_t max_nodes = 0;
   unsigned blocks = 2048;
   hipStream_t stream = nullptr;       // chunks, admission, retirement
@@ -143,6 +147,7 @@ struct hlbm_batch {
   int xcd_split = 1;
   int sr = 0;  // 16-bit storage rounds stochastically
   int64_t chunks = 0, n_done = 0;
+  uint64_t peak_bytes = 0;
   double node_updates = 0, gpu_ms = 0;
 };
 
@@ -268,6 +273,8 @@ int hlbm_batch_create(const hlbm_batch_desc* desc, hlbm_batch** out) {
   if (hipMalloc((void**)&b->d_doms, b->max_doms * sizeof(BatchDomain)) != hipSuccess) return cleanup("domains");
   if (hipMalloc((void**)&b->d_tiles, b->max_tiles * sizeof(BatchTile)) != hipSuccess) return cleanup("tiles");
   if (hipMalloc((void**)&b->d_ntiles, 2 * sizeof(uint32_t)) != hipSuccess) return cleanup("tile count");
+  if (hipMalloc((void**)&b->d_part, 7 * sizeof(double) * (size_t)reduce_parts(0xFFFFFFFFu)) != hipSuccess)
+    return cleanup("reduction partials");
   for (int k = 0; k < 2; k++) {
     if (hipHostMalloc((void**)&b->h_doms_set[k], b->max_doms * sizeof(BatchDomain)) != hipSuccess ||
         hipHostMalloc((void**)&b->h_tiles_set[k], b->max_tiles * sizeof(BatchTile)) != hipSuccess ||
@@ -301,7 +308,7 @@ int hlbm_batch_destroy(hlbm_batch* b) {
   if (b->copy_stream) (void)hipStreamSynchronize(b->copy_stream);
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   for (void* p : {b->arena[0], b->arena[1], (void*)b->flags, (void*)b->bbm, (void*)b->d_doms, (void*)b->d_tiles,
-                  (void*)b->d_ntiles, (void*)b->f_tmp, (void*)b->d_obs[0], (void*)b->d_obs[1]})
+                  (void*)b->d_ntiles, (void*)b->f_tmp, (void*)b->d_part, (void*)b->d_obs[0], (void*)b->d_obs[1]})
     if (p) (void)hipFree(p);
   for (int k = 0; k < 2; k++) {
     for (void* p : {(void*)b->h_doms_set[k], (void*)b->h_tiles_set[k], (void*)b->h_ntiles_set[k], (void*)b->h_obs[k]})
@@ -321,7 +328,8 @@ int hlbm_batch_submit(hlbm_batch* b, const hlbm_solve_desc* s, int64_t* id) {
   if (!b || !s || !id) return bfail(HLBM_ERR_ARG, "NULL argument");
   for (int k = 0; k < 3; k++)
     if (s->nodes[k] < 2) return bfail(HLBM_ERR_ARG, "nodes must be >= 2");
-  if (s->steps < 0 || !(s->tau > 0.0)) return bfail(HLBM_ERR_ARG, "bad steps / tau");
+  if (s->steps < 0) return bfail(HLBM_ERR_ARG, "steps must be >= 0");
+  if (!(s->tau > 0.5)) return bfail(HLBM_ERR_ARG, "tau must be > 0.5 (nu = (tau - 0.5)/3 > 0)");
   if (s->collision != HLBM_BGK && s->collision != HLBM_MRT) return bfail(HLBM_ERR_ARG, "bad collision");
   if ((s->cavity_on[0] || s->cavity_on[1]) && s->periodic[2])
     return bfail(HLBM_ERR_ARG, "a moving wall needs a closed z axis");
@@ -336,6 +344,8 @@ int hlbm_batch_submit(hlbm_batch* b, const hlbm_solve_desc* s, int64_t* id) {
   v.desc = *s;
   v.desc.obstacles = nullptr;
   if (s->n_obstacles > 0) v.obs.assign(s->obstacles, s->obstacles + s->n_obstacles);
+  v.desc.init_f = nullptr;
+  if (s->init_f) v.init_f.assign(s->init_f, s->init_f + (size_t)19 * N);
   v.N = (uint32_t)N;
   v.stride = stride;
   v.f_len = stride * 19;
@@ -421,10 +431,24 @@ static int admit_one(hlbm_batch* b, Solve& s) {
   if (launch_build_bbm(b->flags + s.flag_off, s.dom.nx, s.dom.ny, s.dom.nz, s.dom.per, b->bbm + s.flag_off, st) !=
       hipSuccess)
     return bfail(HLBM_ERR_HIP, "bounce-back mask");
-  int lo[3] = {0, 0, 0}, hi[3] = {s.dom.nx - 1, s.dom.ny - 1, s.dom.nz - 1};
   void* fa = (char*)b->arena[0] + s.f_off * b->esize;
-  if (launch_set_distribution(b->d.storage, fa, s.stride, s.dom.nx, s.dom.ny, lo, hi, s.desc.init_rho, st) != hipSuccess)
-    return bfail(HLBM_ERR_HIP, "set_distribution");
+  if (s.init_f.empty()) {
+    int lo[3] = {0, 0, 0}, hi[3] = {s.dom.nx - 1, s.dom.ny - 1, s.dom.nz - 1};
+    if (launch_set_distribution(b->d.storage, fa, s.stride, s.dom.nx, s.dom.ny, lo, hi, s.desc.init_rho, st) !=
+        hipSuccess)
+      return bfail(HLBM_ERR_HIP, "set_distribution");
+  } else {
+    // host populations, one FP64 plane at a time through f_tmp (stream-ordered),
+    // converted to the storage form (shifted g = f - w below FP64)
+    if (!b->f_tmp && hipMalloc((void**)&b->f_tmp, b->max_nodes * 8) != hipSuccess) return bfail(HLBM_ERR_NOMEM, "tmp");
+    for (int i = 0; i < Q; i++) {
+      void* plane = (char*)fa + (size_t)i * s.stride * b->esize;
+      if (hipMemcpyAsync(b->f_tmp, s.init_f.data() + (size_t)i * s.N, (size_t)s.N * 8, hipMemcpyHostToDevice, st) !=
+              hipSuccess ||
+          launch_convert(b->d.storage, true, plane, b->f_tmp, s.N, i, st) != hipSuccess)
+        return bfail(HLBM_ERR_HIP, "initial populations");
+    }
+  }
   s.parity = 0;
   if (s.desc.steps > 0) {
     if (single_launch(b, s, MODE_COLLIDE, 0) != hipSuccess) return bfail(HLBM_ERR_HIP, "collide");
@@ -458,6 +482,9 @@ static int retire_one(hlbm_batch* b, Solve& s) {
   if (hipMemsetAsync(rho, 0, (size_t)s.N * 8, st) != hipSuccess) return bfail(HLBM_ERR_HIP, "memset");
   if (launch_macro(b->d.storage, fcur, s.stride, s.N, b->flags + s.flag_off, rho, rho + s.N, st) != hipSuccess)
     return bfail(HLBM_ERR_HIP, "macro");
+  // simulation_state of this solve alone (FP64), after rho and u: the failure check
+  if (launch_reduce(rho, rho + s.N, b->flags + s.flag_off, s.N, b->d_part, rho + 4 * (size_t)s.N, st) != hipSuccess)
+    return bfail(HLBM_ERR_HIP, "simulation_state");
   if (s.desc.keep_populations) {  // test and I/O path: read everything back now
     if (!b->f_tmp && hipMalloc((void**)&b->f_tmp, b->max_nodes * 8) != hipSuccess) return bfail(HLBM_ERR_NOMEM, "tmp");
     s.f.resize((size_t)19 * s.N);
@@ -483,8 +510,14 @@ static int poll_retiring(hlbm_batch* b, bool wait, int* finished) {
     Solve& s = b->solves[b->retiring[k]];
     hipError_t q = (wait && k == 0) ? hipEventSynchronize(s.ev) : hipEventQuery(s.ev);
     if (q == hipSuccess) {
+      const double* dstat = (const double*)((const char*)b->arena[s.res_arena] + s.f_off * b->esize) + 4 * (size_t)s.N;
+      if (hipMemcpyAsync(s.stats, dstat, sizeof(s.stats), hipMemcpyDeviceToHost, b->copy_stream) != hipSuccess ||
+          hipStreamSynchronize(b->copy_stream) != hipSuccess)
+        return bfail(HLBM_ERR_HIP, "statistics copy");
+      bool finite = true;
+      for (double v : s.stats) finite = finite && std::isfinite(v);
       s.retiring = false;
-      s.state = HLBM_SOLVE_DONE;
+      s.state = finite ? HLBM_SOLVE_DONE : HLBM_SOLVE_FAILED;
       s.t_done = now_s();
       b->n_done++;
       if (finished) (*finished)++;
@@ -546,6 +579,7 @@ int hlbm_batch_run(hlbm_batch* b, int32_t* n_finished) {
     b->running.push_back(s.id);
     it = b->queue.erase(it);
     b->tiles_dirty = true;
+    b->peak_bytes = std::max<uint64_t>(b->peak_bytes, b->fl_pop.used * b->esize * 2 + b->fl_flag.used * 5);
   }
   // per-domain chunk lengths and the tile table (BGK domains' tiles first)
   const int K = b->d.chunk_steps;
@@ -653,12 +687,26 @@ int hlbm_batch_times(hlbm_batch* b, int64_t id, double* t_submit, double* t_admi
   return HLBM_OK;
 }
 
+int hlbm_batch_stats(hlbm_batch* b, int64_t id, double out[7]) {
+  if (!b || !out) return bfail(HLBM_ERR_ARG, "NULL argument");
+  auto it = b->solves.find(id);
+  if (it == b->solves.end()) return bfail(HLBM_ERR_ARG, "unknown solve id");
+  if (it->second.retiring) {
+    const int rc = poll_retiring(b, false, nullptr);
+    if (rc) return rc;
+  }
+  const Solve& s = it->second;
+  if (s.state != HLBM_SOLVE_DONE && s.state != HLBM_SOLVE_FAILED) return bfail(HLBM_ERR_ARG, "solve not finished");
+  for (int k = 0; k < 7; k++) out[k] = s.stats[k];
+  return HLBM_OK;
+}
+
 int hlbm_batch_result(hlbm_batch* b, int64_t id, double* rho, double* u, double* f) {
   if (!b) return bfail(HLBM_ERR_ARG, "NULL batch");
   auto it = b->solves.find(id);
   if (it == b->solves.end()) return bfail(HLBM_ERR_ARG, "unknown solve id");
   Solve& s = it->second;
-  if (s.state != HLBM_SOLVE_DONE) return bfail(HLBM_ERR_ARG, "solve not finished");
+  if (s.state != HLBM_SOLVE_DONE && s.state != HLBM_SOLVE_FAILED) return bfail(HLBM_ERR_ARG, "solve not finished");
   if (f && !s.desc.keep_populations) return bfail(HLBM_ERR_ARG, "populations were not kept (keep_populations = 0)");
   // the read-out waits for nothing on the main stream: the fields are final
   const double* dres = (const double*)((const char*)b->arena[s.res_arena] + s.f_off * b->esize);
@@ -691,6 +739,7 @@ int hlbm_batch_info_get(hlbm_batch* b, hlbm_batch_info* out) {
   out->budget_bytes = b->d.budget_bytes;
   out->node_updates = b->node_updates;
   out->gpu_ms = b->gpu_ms;
+  out->peak_bytes_in_use = b->peak_bytes;
   return HLBM_OK;
 }
 

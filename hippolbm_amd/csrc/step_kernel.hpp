@@ -755,7 +755,11 @@ __device__ __forceinline__ void apply_program(const StepParams& p, int x, int y,
                   constexpr int qi = Faces::in[face][j];
                   // f*_opp(qi)(x): plane opp(qi) (A/B), plane qi (AA: swapped or natural bounce-back slot)
                   C v = O::ld(fin, (size_t)(mode_aa(MODE) ? qi : OPP(qi)) * st + idx);
-                  if (dim == 2 && p.cav_on[side]) v += C(p.cav[side][j]);
+                  // the natural AA state (AA_EVEN, AA_FIX) holds the value the odd step bounced
+                  // into slot qi, which already carries the moving-wall term when the gather
+                  // adds it (cav_gather); every other state holds the bare f*_opp(qi)(x)
+                  constexpr bool natural = MODE == MODE_AA_EVEN || MODE == MODE_AA_FIX;
+                  if (dim == 2 && p.cav_on[side] && !(natural && p.cav_gather)) v += C(p.cav[side][j]);
                   f[qi] = v;
                 });
               }

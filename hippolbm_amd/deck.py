@@ -54,6 +54,33 @@ class DeckError(ValueError):
     """Malformed or unsupported deck (the reference exits with EXIT_FAILURE or throws)."""
 
 
+# top-level blocks the deck model reads; a block named after the set_distribution
+# operator is read and has no effect (see the module docstring), `configuration`
+# holds Onika's logging / profiling switches
+KNOWN_BLOCKS = ("do_domain", "domain", "set_lbm_parameters", "collision", "pre_stream_bcs", "post_stream_bcs",
+                "boundary_conditions", "set_distributions", "set_obstacles", "start_iteration", "checker_op",
+                "checker", "analysis", "global", "mrt_rates", "includes", "set_distribution", "configuration")
+# batches of the default graph (data/config/*.msp): the run follows their fixed
+# sequence, so a deck that redefines one is refused
+GRAPH_BLOCKS = ("compute_loop", "compute_loop_stop", "initialize", "simulation", "end_iteration", "setup_obstacles",
+                "grid_flavor", "default_grid", "trigger_dump_paraview", "trigger_print_log", "trigger_analysis",
+                "dump_paraview_if_triggered", "print_log_if_triggered", "print_analysis_if_triggered",
+                "write_paraview_first_timestep", "streaming")
+
+
+def check_blocks(doc: dict):
+    for k, v in doc.items():
+        if k in GRAPH_BLOCKS:
+            raise DeckError(f"the deck redefines {k!r} of the default graph (data/config/*.msp); "
+                            "redefinitions are not executed")
+        if isinstance(v, dict):
+            for c in ("rebind", "condition", "loop", "body"):
+                if c in v:
+                    raise DeckError(f"block {k!r} uses Onika graph control ({c!r}), which is not executed")
+        if k not in KNOWN_BLOCKS:
+            raise DeckError(f"unsupported top-level block {k!r}")
+
+
 def _num(v):
     if isinstance(v, bool):
         return v
@@ -200,6 +227,7 @@ class Deck:
 
 def from_dict(doc: dict, name: str = "") -> Deck:
     """Build a Deck from a parsed .msp mapping."""
+    check_blocks(doc)
     dom = None
     for k, v in _items(doc.get("do_domain")):
         if k == "domain":
@@ -295,18 +323,33 @@ def from_dict(doc: dict, name: str = "") -> Deck:
     return d.validate()
 
 
-def load_msp(path: str) -> Deck:
-    """Parse one .msp file (with its ``includes``) into a Deck."""
+def _load_with_includes(path: str, depth: int = 0) -> dict:
+    """A deck file with its ``includes`` (data/config/main-config.msp:11): the listed
+    files, relative to the including file, first and in order; a later file's
+    top-level block replaces an earlier one's; a missing include is an error."""
+    if depth > 16:
+        raise DeckError(f"{path}: includes nested deeper than 16 files")
     with open(path) as fh:
         doc = yaml.safe_load(fh) or {}
+    if not isinstance(doc, dict):
+        raise DeckError(f"{path}: a deck is a map of blocks")
+    incs = doc.get("includes") or []
+    if isinstance(incs, str):
+        incs = [incs]
     merged: dict = {}
-    for inc in doc.get("includes", []) or []:
-        ip = os.path.join(os.path.dirname(path), inc)
-        if os.path.exists(ip):
-            with open(ip) as fh:
-                merged.update(yaml.safe_load(fh) or {})
+    for inc in incs:
+        ip = inc if os.path.isabs(inc) else os.path.join(os.path.dirname(path), inc)
+        if not os.path.exists(ip):
+            raise DeckError(f"{path}: included file {inc!r} not found (looked for {ip})")
+        merged.update(_load_with_includes(ip, depth + 1))
     merged.update(doc)
-    return from_dict(merged, name=os.path.basename(path))
+    merged.pop("includes", None)
+    return merged
+
+
+def load_msp(path: str) -> Deck:
+    """Parse one .msp file (with its ``includes``) into a Deck."""
+    return from_dict(_load_with_includes(path), name=os.path.basename(path))
 
 
 def loads_msp(text: str, name: str = "") -> Deck:

@@ -4,7 +4,11 @@
 // regression checker fails, as the reference does.
 //
 //   hlbm_run deck.msp [--storage f64|f32|f16|bf16] [--pattern ab|aa] [--autotune]
-//                     [--steps N] [--quiet] [--no-output] [--dump-deck]
+//                     [--steps N] [--quiet] [--no-output] [--dump-deck] [--check-deck]
+//
+// --dump-deck prints the deck (with its includes merged) as JSON; --check-deck
+// builds the operator graph (every operator and slot checked) and exits
+// before any device work.
 //
 // Output operators (write_paraview, plane_velocity_profile,
 // plot_line_velocity) write under the deck's output_directory, relative to
@@ -19,12 +23,12 @@
 int main(int argc, char** argv) {
   if (argc < 2) {
     std::fprintf(stderr, "usage: %s deck.msp [--storage f64|f32|f16|bf16] [--pattern ab|aa] [--autotune] "
-                         "[--steps N] [--quiet] [--no-output] [--dump-deck]\n", argv[0]);
+                         "[--steps N] [--quiet] [--no-output] [--dump-deck] [--check-deck]\n", argv[0]);
     return 2;
   }
   std::string path;
   std::string storage = "f64", pattern = "ab";
-  bool autotune = false, quiet = false, dump = false, io = true;
+  bool autotune = false, quiet = false, dump = false, io = true, check_only = false;
   long steps = -1;
   for (int i = 1; i < argc; i++) {
     const std::string a = argv[i];
@@ -35,6 +39,7 @@ int main(int argc, char** argv) {
     else if (a == "--quiet") quiet = true;
     else if (a == "--no-output") io = false;
     else if (a == "--dump-deck") dump = true;
+    else if (a == "--check-deck") check_only = true;
     else if (path.empty()) path = a;
     else {
       std::fprintf(stderr, "unknown argument %s\n", a.c_str());
@@ -42,12 +47,16 @@ int main(int argc, char** argv) {
     }
   }
   try {
-    const msp::Node deck = msp::parse_file(path);
+    const msp::Node deck = msp::load_deck(path);
     if (dump) {
       std::printf("%s\n", msp::to_json(deck).c_str());
       return 0;
     }
     hlbm_host::Simulation sim(deck);
+    if (check_only) {
+      std::printf("deck ok\n");
+      return 0;
+    }
     hlbm_host::Slots& s = sim.slots;
     if (storage == "f64") s.storage = HLBM_F64;
     else if (storage == "f32") s.storage = HLBM_F32;

@@ -340,6 +340,51 @@ Node parse_file(const std::string& path) {
   return parse(ss.str());
 }
 
+namespace {
+
+void merge_top(Node& into, const Node& from) {
+  for (const auto& kv : from.fields) {
+    if (kv.first == "includes") continue;
+    bool replaced = false;
+    for (auto& mine : into.fields)
+      if (mine.first == kv.first) {
+        mine.second = kv.second;
+        replaced = true;
+      }
+    if (!replaced) into.fields.push_back(kv);
+  }
+}
+
+Node load_rec(const std::string& path, int depth) {
+  if (depth > 16) throw ParseError(path + ": includes nested deeper than 16 files");
+  Node doc = parse_file(path);
+  if (doc.is_null()) return doc;
+  if (!doc.is_map()) throw ParseError(path + ": a deck is a map of blocks");
+  const Node* inc = doc.find("includes");
+  if (!inc || inc->is_null()) return doc;
+  const size_t slash = path.find_last_of('/');
+  const std::string dir = slash == std::string::npos ? std::string() : path.substr(0, slash + 1);
+  Node merged;
+  merged.kind = Node::MAP;
+  auto one = [&](const Node& item) {
+    if (!item.is_scalar()) throw ParseError(path + ": includes lists file names");
+    const std::string p = !item.text.empty() && item.text[0] == '/' ? item.text : dir + item.text;
+    std::ifstream probe(p);
+    if (!probe) throw ParseError(path + ": included file '" + item.text + "' not found (looked for " + p + ")");
+    merge_top(merged, load_rec(p, depth + 1));
+  };
+  if (inc->is_seq())
+    for (const auto& item : inc->items) one(item);
+  else
+    one(*inc);
+  merge_top(merged, doc);
+  return merged;
+}
+
+}  // namespace
+
+Node load_deck(const std::string& path) { return load_rec(path, 0); }
+
 std::string to_json(const Node& n) {
   switch (n.kind) {
     case Node::NUL:

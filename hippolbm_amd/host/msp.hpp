@@ -39,6 +39,12 @@ struct ParseError : std::runtime_error {
 
 Node parse(const std::string& text);
 Node parse_file(const std::string& path);
+// A deck with its `includes` (data/config/main-config.msp:11): each listed
+// file, relative to the including file's directory, is read first, in order
+// (recursively, at most 16 levels), and a top-level block of a later file
+// replaces the same-named block of an earlier one; the including file's own
+// blocks come last.  A missing include file is an error.
+Node load_deck(const std::string& path);
 // JSON rendering (scalars as strings), for tests and debugging
 std::string to_json(const Node& n);
 

@@ -471,7 +471,17 @@ std::unique_ptr<OperatorNode> make_bc(int32_t kind, const msp::Node& a, const st
   return op;
 }
 
+// streaming (src/collision/streaming.cu:68-118).  Its `asynchrone: true` form
+// runs nothing in the reference (the overlapped path is commented out,
+// streaming.cu:84-106), i.e. a step without propagation; that is refused here
+// rather than silently streaming.
 struct Streaming : OperatorNode {
+  explicit Streaming(const msp::Node& a) {
+    if (const msp::Node* as = a.find("asynchrone"))
+      if (as->boolean())
+        throw OperatorError("streaming: asynchrone: true skips propagation in the reference "
+                            "(streaming.cu:84-106); not supported");
+  }
   void execute(Slots& s) override { s.streamed = true; }
 };
 
@@ -604,7 +614,10 @@ struct Registrar {
     f->register_factory("neumann", bc(HLBM_BC_NEUMANN, "neumann"));
     f->register_factory("pressure_inlet", bc(HLBM_BC_PRESSURE_INLET, "pressure_inlet"));
     f->register_factory("outflow", bc(HLBM_BC_OUTFLOW, "outflow"));
-    f->register_factory("streaming", plain<Streaming>());
+    f->register_factory("streaming", with_args<Streaming>());
+    // build_traversal (src/grid/build_traversal.cpp:68): the index lists it builds
+    // are the kernels' own coordinate tests here, so it has nothing to do
+    f->register_factory("build_traversal", plain<Nop>());
     f->register_factory("update_ghost", plain<Nop>());  // periodic wrap lives in the step kernel
     f->register_factory("nop", plain<Nop>());
     f->register_factory("simulation_state", plain<SimulationState>());
@@ -681,7 +694,43 @@ std::vector<std::unique_ptr<OperatorNode>> make_list(const msp::Node* list) {
 }
 
 // ---- the simulation graph -------------------------------------------------------
+// Top-level blocks the operator layer reads; everything else a deck could name
+// (a redefinition of the default graph's batches, Onika's rebind / condition /
+// loop control) is refused rather than silently ignored.
+static void check_top_level(const msp::Node& deck) {
+  static const char* known[] = {"do_domain", "set_lbm_parameters", "collision", "pre_stream_bcs", "post_stream_bcs",
+                                "boundary_conditions", "set_distributions", "set_obstacles", "start_iteration",
+                                "checker_op", "checker", "analysis", "global", "mrt_rates", "includes",
+                                // a block named after the set_distribution operator (the reference's
+                                // regression_lbm_diff_pressure.msp pin shows it leaves f unchanged)
+                                "set_distribution",
+                                // Onika's logging / profiling switches: no effect on the results
+                                "configuration"};
+  static const char* graph[] = {"compute_loop", "compute_loop_stop", "initialize", "simulation", "end_iteration",
+                                "setup_obstacles", "grid_flavor", "default_grid", "trigger_dump_paraview",
+                                "trigger_print_log", "trigger_analysis", "dump_paraview_if_triggered",
+                                "print_log_if_triggered", "print_analysis_if_triggered",
+                                "write_paraview_first_timestep", "streaming"};
+  if (!deck.is_map()) return;
+  for (const auto& kv : deck.fields) {
+    const std::string& k = kv.first;
+    for (const char* g : graph)
+      if (k == g)
+        throw OperatorError("the deck redefines '" + k + "' of the default graph (data/config/*.msp); the operator "
+                            "layer runs that graph's fixed sequence and does not execute redefinitions");
+    if (kv.second.is_map())
+      for (const char* c : {"rebind", "condition", "loop", "body"})
+        if (kv.second.find(c))
+          throw OperatorError("block '" + k + "' uses Onika graph control ('" + c +
+                              "'), which the operator layer does not execute");
+    bool ok = false;
+    for (const char* n : known) ok = ok || k == n;
+    if (!ok) throw OperatorError("unsupported top-level block '" + k + "'");
+  }
+}
+
 Simulation::Simulation(const msp::Node& deck) {
+  check_top_level(deck);
   do_domain = make_list(deck.find("do_domain"));
   set_lbm_parameters = make_list(deck.find("set_lbm_parameters"));
   const msp::Node* coll = deck.find("collision");
@@ -770,7 +819,7 @@ void Simulation::compute_loop() {
     run(collision, s);
     s.stage = HLBM_STAGE_PRE_STREAM;
     run(pre_stream_bcs, s);
-    Streaming().execute(s);
+    Streaming(msp::Node{}).execute(s);
     s.stage = HLBM_STAGE_POST_STREAM;
     run(post_stream_bcs, s);
     s.stage = HLBM_STAGE_BOUNDARY;

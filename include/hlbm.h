@@ -96,7 +96,7 @@ typedef struct {
   int32_t pattern;     /* enum hlbm_pattern                                                    */
   int32_t collision;   /* enum hlbm_collision                                                  */
   int32_t mrt_custom;  /* 0: hippoLBM's hard-coded MRT rates (mrt.hpp:43-47); 1: mrt_rates    */
-  double tau;          /* LBMParameters::tau_                                                  */
+  double tau;          /* LBMParameters::tau_; must be > 0.5 (nu = (tau - 0.5)/3 > 0)          */
   double fext[3];      /* LBMParameters::Fext_ (LBM units)                                     */
   double dx;           /* grid spacing, used by obstacle rasterization                         */
   double mrt_rates[10];/* s1, s2, s4, s9, s10, s13, s16, weps, wepsj, wxx                     */
@@ -259,22 +259,31 @@ typedef struct {
   int32_t cavity_on[2]; /* cavity_z_0 / cavity_z_l moving walls (z must be closed)   */
   int32_t keep_populations; /* also return the final populations (FP64, node layout) */
   int32_t mrt_custom;
-  double tau;
+  double tau;           /* > 0.5                                                     */
   double fext[3];
   double dx;
   double init_rho;      /* initial state f_i = init_rho * w_i                        */
   double cavity_u[2][3];
   double mrt_rates[10];
   const hlbm_obstacle* obstacles; /* copied at submit                                */
+  const double* init_f; /* optional initial populations, FP64 node layout SoA        *
+                         * (f[i*N + (z*ny + y)*nx + x]); NULL = init_rho * w_i;      *
+                         * copied at submit                                          */
 } hlbm_solve_desc;
 
-enum hlbm_solve_state { HLBM_SOLVE_QUEUED = 0, HLBM_SOLVE_RUNNING = 1, HLBM_SOLVE_DONE = 2 };
+/* A solve ends DONE, or FAILED when its final state holds a NaN or an infinity:
+ * at retirement the simulation_state reduction (src/io/simulation_state.cu:73-98,
+ * FP64: sum rho, min/max |u|, momentum) runs over the solve alone, and a
+ * non-finite sum marks that solve only; the other solves of the batch are not
+ * affected.  A FAILED solve's fields can still be read with hlbm_batch_result. */
+enum hlbm_solve_state { HLBM_SOLVE_QUEUED = 0, HLBM_SOLVE_RUNNING = 1, HLBM_SOLVE_DONE = 2, HLBM_SOLVE_FAILED = 3 };
 
 typedef struct {
   int64_t queued, running, done, chunks;
   uint64_t bytes_in_use, budget_bytes;
   double node_updates; /* lattice node updates performed so far                     */
   double gpu_ms;       /* device time of all chunks (HIP events)                    */
+  uint64_t peak_bytes_in_use; /* the largest bytes_in_use after any admission        */
 } hlbm_batch_info;
 
 int hlbm_batch_create(const hlbm_batch_desc* desc, hlbm_batch** out);
@@ -286,7 +295,13 @@ int hlbm_batch_run(hlbm_batch* b, int32_t* n_finished);
 int hlbm_batch_status(hlbm_batch* b, int64_t id, int32_t* state, int64_t* steps_done);
 /* Seconds on the library's steady clock: submission, admission, completion. */
 int hlbm_batch_times(hlbm_batch* b, int64_t id, double* t_submit, double* t_admit, double* t_done);
-/* Copy a finished solve's rho (N), u (3N) and, if kept, f (19N), then forget it. */
+/* A finished (DONE or FAILED) solve's simulation_state: out[0] sum rho over the
+ * nodes, out[1] min |u|, out[2] max |u|, out[3..5] sum rho u, out[6] fluid nodes
+ * (src/io/simulation_state.cu:73-98; walls hold rho = 0, u = 0). */
+int hlbm_batch_stats(hlbm_batch* b, int64_t id, double out[7]);
+/* Copy a finished solve's rho (N), u (3N) and, if kept, f (19N), then forget it
+ * (any of the three pointers may be NULL: a call with all three NULL only
+ * releases the solve). */
 int hlbm_batch_result(hlbm_batch* b, int64_t id, double* rho, double* u, double* f);
 int hlbm_batch_info_get(hlbm_batch* b, hlbm_batch_info* out);
 /* The library's steady clock, seconds (same clock as hlbm_batch_times). */

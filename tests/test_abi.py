@@ -47,6 +47,7 @@ def test_struct_layout_matches_header(tmp_path):
         ' offsetof(hlbm_obstacle, q));\n'
         'printf("%zu %zu %zu %zu\\n", sizeof(hlbm_solve_desc), offsetof(hlbm_solve_desc, tau),'
         ' offsetof(hlbm_solve_desc, cavity_u), offsetof(hlbm_solve_desc, obstacles));\n'
+        'printf("%zu\\n", offsetof(hlbm_solve_desc, init_f));\n'
         'printf("%zu %zu %zu\\n", sizeof(hlbm_batch_desc), sizeof(hlbm_batch_info), offsetof(hlbm_batch_info, gpu_ms));\n'
         'printf("%zu %zu\\n", sizeof(hlbm_tune_info), offsetof(hlbm_tune_info, ms_default));\n'
         "return 0;}\n")
@@ -65,9 +66,10 @@ def test_struct_layout_matches_header(tmp_path):
     sd = [int(v) for v in lines[3].split()]
     S = bt.SolveDesc
     assert sd == [ctypes.sizeof(S), S.tau.offset, S.cavity_u.offset, S.obstacles.offset]
-    bd = [int(v) for v in lines[4].split()]
+    assert int(lines[4]) == S.init_f.offset
+    bd = [int(v) for v in lines[5].split()]
     assert bd == [ctypes.sizeof(bt.BatchDesc), ctypes.sizeof(bt.BatchInfo), bt.BatchInfo.gpu_ms.offset]
-    tu = [int(v) for v in lines[5].split()]
+    tu = [int(v) for v in lines[6].split()]
     assert tu == [ctypes.sizeof(_lib.TuneInfo), _lib.TuneInfo.ms_default.offset]
 
 
@@ -77,6 +79,18 @@ def test_null_arguments_are_rejected_without_a_device():
     assert L.hlbm_device_count(None) == -1
     assert b"NULL" in L.hlbm_last_error()
     assert L.hlbm_domain_destroy(None) == 0
+
+
+def test_tau_at_or_below_one_half_is_rejected_without_a_device():
+    # nu = (tau - 1/2)/3 <= 0: refused before any device call
+    L = _lib.lib()
+    d = _lib.DomainDesc()
+    for k in range(3):
+        d.nodes[k] = 8
+    d.tau = 0.5
+    h = ctypes.c_void_p()
+    assert L.hlbm_domain_create(ctypes.byref(d), ctypes.byref(h)) == -1
+    assert b"tau must be > 0.5" in L.hlbm_last_error()
 
 
 def test_enums_match_header():

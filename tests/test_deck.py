@@ -76,3 +76,25 @@ def test_poiseuille_deck_force_in_lattice_units():
     d = workloads.poiseuille((32, 8, 16))
     assert d.params["Fext"][0] == pytest.approx(1e-6)
     assert d.periodic == (True, True, False)
+
+
+def test_python_reader_refuses_dangling_includes_and_graph_blocks(tmp_path):
+    from conftest import REGRESSION_DECKS
+    base = open(REGRESSION_DECKS[0]).read()
+    p = tmp_path / "d.msp"
+    p.write_text("includes: [nope.msp]\n" + base)
+    with pytest.raises(dk.DeckError, match="not found"):
+        dk.load_msp(str(p))
+    # an included block is used when the deck has none of that name; the deck's own block replaces it otherwise
+    import yaml
+    doc = yaml.safe_load(base)
+    doc.pop("global")
+    (tmp_path / "inc.msp").write_text("global:\n   simulation_end_iteration: 7\n")
+    p.write_text("includes: [inc.msp]\n" + yaml.safe_dump(doc, default_flow_style=None))
+    assert dk.load_msp(str(p)).end_iteration == 7
+    p.write_text("includes: [inc.msp]\n" + base)
+    assert dk.load_msp(str(p)).end_iteration == 100
+    for block in ("compute_loop:\n  loop: true\n  body:\n    - macro_variables\n",
+                  "my_batch:\n  condition: x\n  body:\n    - log\n", "mystery_block: 3\n"):
+        with pytest.raises(dk.DeckError):
+            dk.loads_msp(base + "\n" + block)

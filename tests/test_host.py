@@ -62,3 +62,64 @@ def test_required_analysis_slot_is_checked_before_device_work(tmp_path):
     r = subprocess.run([RUN, str(deck)], capture_output=True, text=True, cwd=tmp_path)
     assert r.returncode != 0 and "the dimension slot is required" in r.stdout
     assert not (tmp_path / "hippoLBMOutputDir").exists()
+
+
+def _check(tmp_path, text, name="deck.msp"):
+    deck = tmp_path / name
+    deck.write_text(text)
+    return subprocess.run([RUN, str(deck), "--check-deck"], capture_output=True, text=True, cwd=tmp_path)
+
+
+def test_includes_are_resolved_relative_to_the_deck(tmp_path):
+    base = open(DECK_FILES[0]).read()
+    sub = tmp_path / "common"
+    sub.mkdir()
+    (sub / "params.msp").write_text("set_lbm_parameters:\n  - lbm_parameters:\n     nuth: 1e-3\n")
+    r = _check(tmp_path, "includes:\n  - common/params.msp\n" + base)
+    assert r.returncode == 0 and "deck ok" in r.stdout, r.stdout + r.stderr
+    out = subprocess.run([RUN, str(tmp_path / "deck.msp"), "--dump-deck"], capture_output=True, text=True,
+                         check=True).stdout
+    tree = json.loads(out)
+    assert "includes" not in tree
+    # the deck's own set_lbm_parameters block replaces the included one
+    assert same(tree["set_lbm_parameters"], yaml.safe_load(base)["set_lbm_parameters"])
+
+
+def test_included_block_is_used_when_the_deck_lacks_it(tmp_path):
+    (tmp_path / "params.msp").write_text("set_lbm_parameters:\n  - lbm_parameters:\n     nuth: 1e-3\n")
+    doc = yaml.safe_load(open(DECK_FILES[0]).read())
+    doc.pop("set_lbm_parameters")
+    (tmp_path / "deck.msp").write_text("includes: [params.msp]\n" + yaml.safe_dump(doc, default_flow_style=None))
+    tree = json.loads(subprocess.run([RUN, str(tmp_path / "deck.msp"), "--dump-deck"], capture_output=True,
+                                     text=True, check=True).stdout)
+    assert float(tree["set_lbm_parameters"][0]["lbm_parameters"]["nuth"]) == 1e-3
+
+
+def test_dangling_include_is_refused(tmp_path):
+    r = _check(tmp_path, "includes:\n  - missing.msp\n" + open(DECK_FILES[0]).read())
+    assert r.returncode != 0 and "missing.msp" in (r.stdout + r.stderr) and "not found" in (r.stdout + r.stderr)
+
+
+@pytest.mark.parametrize("block", [
+    "compute_loop:\n  loop: true\n  condition: compute_loop_continue\n  body:\n    - macro_variables\n",
+    "end_iteration:\n  body:\n    - trigger_print_log\n",
+    "my_trigger:\n  rebind: { freq: simulation_paraview_freq , result: t }\n  body:\n    - nth_timestep: { first: false }\n",
+    "print_log_if_triggered:\n  condition: trigger_print_log\n  body:\n    - log\n",
+    "write_vtk_slices:\n  - nop\n",
+])
+def test_graph_redefinitions_are_refused(tmp_path, block):
+    r = _check(tmp_path, open(DECK_FILES[0]).read() + "\n" + block)
+    assert r.returncode != 0
+    assert "default graph" in r.stdout or "graph control" in r.stdout or "unsupported top-level block" in r.stdout
+
+
+def test_asynchronous_streaming_is_refused(tmp_path):
+    txt = open(DECK_FILES[0]).read().replace("pre_stream_bcs:\n", "pre_stream_bcs:\n  - streaming:\n     asynchrone: true\n", 1)
+    assert "asynchrone" in txt
+    r = _check(tmp_path, txt)
+    assert r.returncode != 0 and "asynchrone" in r.stdout
+
+
+def test_build_traversal_is_a_known_operator(tmp_path):
+    r = _check(tmp_path, open(DECK_FILES[0]).read() + "\nstart_iteration:\n  - build_traversal\n")
+    assert r.returncode == 0, r.stdout
