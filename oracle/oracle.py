@@ -241,7 +241,11 @@ class OracleRun:
         if threads:
             lib().orc_set_threads(int(threads))
         self.deck = deck
-        self.p = deck.params
+        # lbm_parameters from the deck's raw slot values through the oracle's own
+        # restatement (slot defaults: lbm_parameters.cpp:48-56), not the engine's
+        lb = deck.lbm
+        self.p = lbm_parameters(deck.dx, lb.get("nuth", 1e-4), lb.get("celerity", 1.0), lb.get("tau"),
+                                lb.get("dt", 0.0), lb.get("Fext", (0.0, 0.0, 0.0)))
         self.o = Oracle(deck.nodes, deck.periodic, deck.dx)
         o = self.o
         o.init_obst()

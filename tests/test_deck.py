@@ -98,3 +98,32 @@ def test_python_reader_refuses_dangling_includes_and_graph_blocks(tmp_path):
                   "my_batch:\n  condition: x\n  body:\n    - log\n", "mystery_block: 3\n"):
         with pytest.raises(dk.DeckError):
             dk.loads_msp(base + "\n" + block)
+
+
+def test_engine_parameters_equal_the_oracle_restatement(decks_dir):
+    """Deck.params (the engine's lbm_parameters) against the oracle's own C
+    restatement of lbm_parameters.cpp:73-99, bit for bit, for the regression
+    decks and for each of the three definitions (dt, tau, celerity)."""
+    import glob
+    import os
+
+    import oracle as orc
+    from hippolbm_amd import deck as dk
+    cases = [dk.load_msp(p) for p in sorted(glob.glob(os.path.join(decks_dir, "regression_*.msp")))]
+    assert len(cases) == 7
+    for lbm in ({"dt": 2.5e-4, "nuth": 1e-3, "Fext": [0.3, 0, -1e-2]},
+                {"tau": 0.65, "nuth": 1e-3, "Fext": [1.3, 0, 0]},
+                {"celerity": 10.0, "nuth": 1e-3, "Fext": [0, 2.0, 0]},
+                {"nuth": 1e-4}):
+        d = dk.loads_msp("do_domain:\n  - domain:\n     bounds: [[0,0,0],[0.3,0.2,0.1]]\n"
+                         "     cell_dims: [30, 20, 10]\n     periodic: [true, false, false]\n", "params")
+        d.lbm = dict(lbm)
+        cases.append(d)
+    for d in cases:
+        lb = d.lbm
+        want = orc.lbm_parameters(d.dx, lb.get("nuth", 1e-4), lb.get("celerity", 1.0), lb.get("tau"),
+                                  lb.get("dt", 0.0), lb.get("Fext", (0.0, 0.0, 0.0)))
+        got = d.params
+        for k in ("tau", "nu", "dtLB", "celerity"):
+            assert got[k] == want[k], (d.name, k, got[k], want[k])
+        assert tuple(got["Fext"]) == tuple(want["Fext"]), d.name

@@ -283,6 +283,71 @@ def test_mrt_equals_bgk_under_lbgk_rates():
     assert np.abs(a.node_f() - b.node_f()).max() < 1e-13
 
 
+def dhumieres_mrt(f, tau, F, weps=0.0, wepsj=-475.0 / 63.0, wxx=0.0, s1=1.19, s2=1.4, s4=1.2, s10=1.4,
+                  s16=1.98):
+    """Matrix-form D3Q19 MRT collision of d'Humieres, Ginzburg, Krafczyk,
+    Lallemand and Luo, Phil. Trans. R. Soc. A 360 (2002) 437-451: the moment
+    basis built from its polynomial definitions in c, M^-1 by numerical
+    inversion, the equilibria with rho0 = 1, and the relaxation rates the paper
+    recommends (the defaults here, which mrt.hpp:43-47 quotes); the body force
+    enters as 3 rho w_i (c_i . F) after the collision."""
+    c2 = (C * C).sum(1)
+    cx, cy, cz = C[:, 0], C[:, 1], C[:, 2]
+    rows = [np.ones(19), 19 * c2 - 30, (21 * c2 * c2 - 53 * c2 + 24) / 2,
+            cx, (5 * c2 - 9) * cx, cy, (5 * c2 - 9) * cy, cz, (5 * c2 - 9) * cz,
+            3 * cx * cx - c2, (3 * c2 - 5) * (3 * cx * cx - c2),
+            cy * cy - cz * cz, (3 * c2 - 5) * (cy * cy - cz * cz),
+            cx * cy, cy * cz, cx * cz,
+            (cy * cy - cz * cz) * cx, (cz * cz - cx * cx) * cy, (cx * cx - cy * cy) * cz]
+    M = np.array(rows, dtype=np.float64)
+    m = np.einsum("kq,q...->k...", M, f)
+    rho, jx, jy, jz = m[0], m[3], m[5], m[7]
+    jj = jx * jx + jy * jy + jz * jz
+    pxx = 2 * jx * jx - (jy * jy + jz * jz)
+    pww = jy * jy - jz * jz
+    meq = np.zeros_like(m)
+    meq[0] = rho
+    meq[1] = -11 * rho + 19 * jj
+    meq[2] = weps * rho + wepsj * jj
+    meq[3], meq[5], meq[7] = jx, jy, jz
+    meq[4], meq[6], meq[8] = -2 / 3 * jx, -2 / 3 * jy, -2 / 3 * jz
+    meq[9], meq[10] = pxx, wxx * pxx
+    meq[11], meq[12] = pww, wxx * pww
+    meq[13], meq[14], meq[15] = jx * jy, jy * jz, jx * jz
+    s9 = 1 / tau
+    S = np.array([0, s1, s2, 0, s4, 0, s4, 0, s4, s9, s10, s9, s10, s9, s9, s9, s16, s16, s16])
+    mpost = m - S.reshape((19,) + (1,) * (m.ndim - 1)) * (m - meq)
+    fpost = np.einsum("qk,k...->q...", np.linalg.inv(M), mpost)
+    ef = C @ np.asarray(F, dtype=np.float64)
+    return fpost + 3 * rho[None] * (W * ef).reshape((19,) + (1,) * rho.ndim)
+
+
+def test_mrt_default_rates_match_the_published_model():
+    # random off-equilibrium populations: every moment is away from equilibrium,
+    # so each rate (s1, s2, s4, s16 and the nu rate) and each equilibrium weight
+    # (weps = 0, wepsj = -475/63, wxx = 0) shows in the result
+    rng = np.random.default_rng(11)
+    n = (5, 4, 3)
+    tau, F = 0.77, (2e-5, -1e-5, 3e-6)
+    f = feq(1 + rng.uniform(-0.02, 0.02, n[::-1]), rng.uniform(-0.06, 0.06, (3,) + n[::-1]))
+    f = f + rng.uniform(-2e-3, 2e-3, f.shape)
+    o = _oracle_with(f)
+    o.macro_variables()
+    o.mrt(tau, F)
+    want = dhumieres_mrt(f, tau, F)
+    assert np.abs(o.node_f() - want).max() < 1e-14
+    # a wrong rate or weight is visible: each one perturbed alone moves the result
+    for k, v in {"s1": 1.2, "s2": 1.41, "s4": 1.21, "s10": 1.39, "s16": 1.97, "weps": 0.01,
+                 "wepsj": -475.0 / 63.0 + 0.01, "wxx": 0.01}.items():
+        other = dhumieres_mrt(f, tau, F, **{k: v})
+        assert np.abs(other - want).max() > 1e-9, k
+    # custom rates [s1 s2 s4 s9 s10 s13 s16 weps wepsj wxx] reproduce the defaults when set to them
+    o2 = _oracle_with(f)
+    o2.macro_variables()
+    o2.mrt(tau, F, rates=[1.19, 1.4, 1.2, 1 / tau, 1.4, 1 / tau, 1.98, 0.0, -475.0 / 63.0, 0.0])
+    assert np.abs(o2.node_f() - want).max() < 1e-14
+
+
 def test_mrt_conserves_mass_and_momentum():
     rng = np.random.default_rng(3)
     n = (4, 4, 4)
