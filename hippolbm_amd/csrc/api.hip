@@ -61,6 +61,18 @@ struct hlbm_domain {
   void* oflow = nullptr;    // outflow values captured before in-place AA steps
   uint32_t oflow_n = 0;     // nodes per captured plane
   bool bbm_dirty = true;
+  // ghost flags of the periodic axes' first ghost layer (padded grid, see
+  // ghost_count); allocated when some axis is periodic
+  uint8_t* gflags = nullptr;
+  // nodes next to a wall across a periodic face (special_kernel)
+  uint32_t nspec = 0, spec_cap = 0;
+  uint32_t* spec_idx = nullptr;
+  uint32_t* spec_mask = nullptr;
+  uint32_t* spec_count = nullptr;
+  void* spec_val = nullptr;      // previous-step values, C type, [i * nspec + k]
+  void* spec_scratch = nullptr;  // the same size, for the autotuner's steps
+  std::vector<uint32_t> spec_host_idx, spec_host_mask;
+  uint64_t plan_serial = 1, spec_checked = 0;
   StepParams plan{};
   dim3 block{128, 2, 1};
   int xcd_remap = 1;
@@ -107,6 +119,10 @@ static StepParams base_params(hlbm_domain* h) {
   if (!h->any_wall) p.wall_bb = 0;
   p.any_wall = h->any_wall ? 1 : 0;
   p.bbm = h->bbm;
+  p.nspec = p.wall_bb ? h->nspec : 0;
+  p.spec_idx = h->spec_idx;
+  p.spec_mask = h->spec_mask;
+  p.spec_val = h->spec_val;
   p.force_wide = h->force_wide || h->variant == 1 ? 1 : 0;
   p.boundary_first = h->boundary_first;
   p.pair32 = ((uint64_t)h->stride + h->N) * h->esize < 0xFFFFFFFFull ? 1 : 0;
@@ -116,9 +132,96 @@ static StepParams base_params(hlbm_domain* h) {
   return p;
 }
 
+static bool any_periodic(const hlbm_domain* h) {
+  return h->d.periodic[0] || h->d.periodic[1] || h->d.periodic[2];
+}
+
+static size_t compute_size(const hlbm_domain* h) { return h->d.storage == HLBM_F64 ? 8 : 4; }
+
+// Nodes next to a wall across a periodic face (k_find_special), listed after
+// the bounce-back mask is built.  The listed values are written by the next
+// collision kernel; the flags only change with the state at a pre-collision
+// phase (see materialize_for_change), so no listed value is read before.
+static int build_special(hlbm_domain* h, hipStream_t s) {
+  h->nspec = 0;
+  h->spec_host_idx.clear();
+  h->spec_host_mask.clear();
+  h->spec_checked = 0;
+  if (!any_periodic(h) || !h->gflags) return HLBM_OK;
+  if (!h->spec_idx) {
+    uint64_t cap = 0;
+    const uint64_t n[3] = {(uint64_t)h->nx, (uint64_t)h->ny, (uint64_t)h->nz};
+    for (int d = 0; d < 3; d++)
+      if (h->d.periodic[d]) cap += 2 * n[(d + 1) % 3] * n[(d + 2) % 3];
+    cap = std::min<uint64_t>(cap, h->N);
+    if (hipMalloc((void**)&h->spec_idx, cap * 4) != hipSuccess ||
+        hipMalloc((void**)&h->spec_mask, cap * 4) != hipSuccess ||
+        hipMalloc((void**)&h->spec_count, 4) != hipSuccess)
+      return fail(HLBM_ERR_NOMEM, "periodic-face wall list");
+    h->spec_cap = (uint32_t)cap;
+    h->bytes += cap * 8 + 4;
+  }
+  const int per[3] = {h->d.periodic[0] ? 1 : 0, h->d.periodic[1] ? 1 : 0, h->d.periodic[2] ? 1 : 0};
+  uint32_t count = 0;
+  HIP_TRY(hipMemsetAsync(h->spec_count, 0, 4, s));
+  HIP_TRY(launch_find_special(h->flags, h->gflags, h->nx, h->ny, h->nz, per, h->bbm, h->spec_idx, h->spec_mask,
+                              h->spec_count, h->spec_cap, s));
+  HIP_TRY(hipMemcpyAsync(&count, h->spec_count, 4, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (count > h->spec_cap) return fail(HLBM_ERR_HIP, "periodic-face wall list overflow");
+  if (count == 0) return HLBM_OK;
+  h->spec_host_idx.resize(count);
+  h->spec_host_mask.resize(count);
+  HIP_TRY(hipMemcpy(h->spec_host_idx.data(), h->spec_idx, (size_t)count * 4, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(h->spec_host_mask.data(), h->spec_mask, (size_t)count * 4, hipMemcpyDeviceToHost));
+  for (void** q : {&h->spec_val, &h->spec_scratch})
+    if (*q) {
+      (void)hipFree(*q);
+      *q = nullptr;
+    }
+  const size_t vbytes = (size_t)count * Q * compute_size(h);
+  if (hipMalloc(&h->spec_val, vbytes) != hipSuccess || hipMalloc(&h->spec_scratch, vbytes) != hipSuccess)
+    return fail(HLBM_ERR_NOMEM, "periodic-face wall values");
+  HIP_TRY(hipMemset(h->spec_val, 0, vbytes));
+  h->nspec = count;
+  return HLBM_OK;
+}
+
+// The previous-step values stand for the ghost's populations only while no
+// boundary operator treats the node differently from its ghost: the ghost
+// lies outside the planes of the crossed axes, the reference's plane operators
+// (pre/post_bounce_back, cavity, lid, neumann) and set_distribution boxes act
+// on ghosts by their own positions and with their own stale inputs.  Those
+// combinations, and a wall ghost over a fluid image, are refused.
+static int check_special(hlbm_domain* h) {
+  if (!h->nspec || h->spec_checked == h->plan_serial) return HLBM_OK;
+  const StepParams& P = h->plan;
+  for (size_t k = 0; k < h->spec_host_idx.size(); k++) {
+    if (h->spec_host_mask[k] >> 31)
+      return fail(HLBM_ERR_UNSUPPORTED,
+                  "wall_bounce_back: a wall across a periodic face whose ghost node, rasterized at its own "
+                  "position, is a wall while its periodic image is fluid (hippoLBM then streams the wall's own "
+                  "populations, which this engine does not keep)");
+    const uint32_t idx = h->spec_host_idx[k];
+    const int x = (int)(idx % (uint32_t)h->nx), y = (int)(idx / (uint32_t)h->nx % (uint32_t)h->ny),
+              z = (int)(idx / ((uint32_t)h->nx * (uint32_t)h->ny));
+    const int planes = (z == 0 ? 1 : 0) | (z == h->nz - 1 ? 2 : 0) | (y == 0 ? 4 : 0) | (y == h->ny - 1 ? 8 : 0) |
+                       (x == 0 ? 16 : 0) | (x == h->nx - 1 ? 32 : 0);
+    int closed = 0;  // planes of the non-periodic axes
+    for (int pl = 0; pl < 6; pl++)
+      if (!h->d.periodic[plane_dim(pl)]) closed |= 1 << pl;
+    if ((planes & P.prog_planes) || (P.plane_bb && (planes & closed)) || P.nset)
+      return fail(HLBM_ERR_UNSUPPORTED,
+                  "wall_bounce_back: a node next to a wall across a periodic face lies on a plane of a boundary "
+                  "operator or under start_iteration set_distribution");
+  }
+  h->spec_checked = h->plan_serial;
+  return HLBM_OK;
+}
+
 // The bulk kernel's bounce-back mask, rebuilt after the flags changed.
 static int ensure_bbm(hlbm_domain* h, hipStream_t s) {
-  if (!h->bbm_dirty && h->bbm) return HLBM_OK;
+  if (!h->bbm_dirty && h->bbm) return check_special(h);
   if (!h->bbm) {
     if (hipMalloc((void**)&h->bbm, (size_t)h->N * 4) != hipSuccess) return fail(HLBM_ERR_NOMEM, "bounce-back mask");
     h->bytes += (uint64_t)h->N * 4;
@@ -126,8 +229,10 @@ static int ensure_bbm(hlbm_domain* h, hipStream_t s) {
   const int per[3] = {h->d.periodic[0] ? 1 : 0, h->d.periodic[1] ? 1 : 0, h->d.periodic[2] ? 1 : 0};
   hipError_t e = launch_build_bbm(h->flags, h->nx, h->ny, h->nz, per, h->bbm, s);
   if (e != hipSuccess) return fail(HLBM_ERR_HIP, std::string("bounce-back mask: ") + hipGetErrorString(e));
+  const int rc = build_special(h, s);
+  if (rc) return rc;
   h->bbm_dirty = false;
-  return HLBM_OK;
+  return check_special(h);
 }
 
 static int launch(hlbm_domain* h, int mode, bool write_macro, hipStream_t s, void* out = nullptr,
@@ -190,6 +295,11 @@ static int materialize(hlbm_domain* h, hipStream_t s) {
   return HLBM_OK;
 }
 
+// Flags and the boundary plan change only at a pre-collision phase: the pending
+// streaming of the last step belongs to the old ones (and the periodic-face
+// wall list, rebuilt with the flags, is filled by the next collision).
+static int materialize_for_change(hlbm_domain* h) { return h->phase == PHASE_PRE ? HLBM_OK : materialize(h, 0); }
+
 extern "C" {
 
 const char* hlbm_last_error(void) { return last_error().c_str(); }
@@ -244,7 +354,7 @@ int hlbm_domain_create(const hlbm_domain_desc* desc, hlbm_domain** out) {
   const size_t fbytes = (size_t)h->stride * Q * h->esize;
   auto cleanup = [&](const std::string& what) {
     for (void* p : {h->buf[0], h->buf[1], (void*)h->flags, (void*)h->m0, (void*)h->m1, (void*)h->red_part,
-                    (void*)h->red_out, (void*)h->tmp})
+                    (void*)h->red_out, (void*)h->tmp, (void*)h->gflags})
       if (p) (void)hipFree(p);
     delete h;
     return fail(HLBM_ERR_NOMEM, "device allocation failed: " + what);
@@ -257,7 +367,10 @@ int hlbm_domain_create(const hlbm_domain_desc* desc, hlbm_domain** out) {
   if (hipMalloc((void**)&h->m1, N64 * 24) != hipSuccess) return cleanup("u");
   if (hipMalloc((void**)&h->red_part, 7 * 8 * 1024) != hipSuccess) return cleanup("reduction");
   if (hipMalloc((void**)&h->red_out, 7 * 8) != hipSuccess) return cleanup("reduction");
-  h->bytes = (aa ? 1 : 2) * fbytes + N64 * (1 + 8 + 24) + 7 * 8 * 1025;
+  const size_t gbytes = ghost_count(h->nx, h->ny, h->nz);
+  if (any_periodic(h) && (hipMalloc((void**)&h->gflags, gbytes) != hipSuccess || hipMemset(h->gflags, 0, gbytes) != hipSuccess))
+    return cleanup("ghost flags");
+  h->bytes = (aa ? 1 : 2) * fbytes + N64 * (1 + 8 + 24) + 7 * 8 * 1025 + (h->gflags ? gbytes : 0);
   if (hipMemset(h->buf[0], 0, fbytes) != hipSuccess || (!aa && hipMemset(h->buf[1], 0, fbytes) != hipSuccess) ||
       hipMemset(h->flags, 0, N64) != hipSuccess || hipMemset(h->m0, 0, N64 * 8) != hipSuccess ||
       hipMemset(h->m1, 0, N64 * 24) != hipSuccess)
@@ -282,7 +395,8 @@ int hlbm_domain_destroy(hlbm_domain* h) {
   if (!h) return HLBM_OK;
   (void)hipDeviceSynchronize();
   for (void* p : {h->buf[0], h->buf[1], (void*)h->flags, (void*)h->m0, (void*)h->m1, (void*)h->red_part,
-                  (void*)h->red_out, (void*)h->tmp, (void*)h->bbm, h->oflow})
+                  (void*)h->red_out, (void*)h->tmp, (void*)h->bbm, h->oflow, (void*)h->gflags, (void*)h->spec_idx,
+                  (void*)h->spec_mask, (void*)h->spec_count, h->spec_val, h->spec_scratch})
     if (p) (void)hipFree(p);
   delete h;
   return HLBM_OK;
@@ -290,7 +404,7 @@ int hlbm_domain_destroy(hlbm_domain* h) {
 
 int hlbm_domain_bytes(const hlbm_domain* h, uint64_t* bytes) {
   if (!h || !bytes) return fail(HLBM_ERR_ARG, "NULL argument");
-  *bytes = h->bytes + (h->tmp ? (uint64_t)h->N * 8 : 0);
+  *bytes = h->bytes + (h->tmp ? (uint64_t)h->N * 8 : 0) + (uint64_t)h->nspec * Q * compute_size(h) * 2;
   return HLBM_OK;
 }
 
@@ -305,6 +419,10 @@ int hlbm_set_params(hlbm_domain* h, double tau, const double fext[3]) {
 // Compile the three operator lists into the kernel's boundary plan.
 int hlbm_set_bcs(hlbm_domain* h, const hlbm_bc* list, int32_t n) {
   if (!h || (n > 0 && !list) || n < 0) return fail(HLBM_ERR_ARG, "bad bc list");
+  {
+    const int rc = materialize_for_change(h);
+    if (rc) return rc;
+  }
   StepParams P{};
   bool pre_bb = false, post_bb = false;
   bool cav_after_pre[2] = {false, false};
@@ -457,12 +575,18 @@ int hlbm_set_bcs(hlbm_domain* h, const hlbm_bc* list, int32_t n) {
     h->oflow_n = (uint32_t)pl;
   }
   h->plan = P;
+  h->plan_serial++;
   return HLBM_OK;
 }
 
 int hlbm_init_obst(hlbm_domain* h) {
   if (!h) return fail(HLBM_ERR_ARG, "NULL domain");
+  {
+    const int rc = materialize_for_change(h);
+    if (rc) return rc;
+  }
   HIP_TRY(hipMemset(h->flags, 0, h->N));
+  if (h->gflags) HIP_TRY(hipMemset(h->gflags, 0, ghost_count(h->nx, h->ny, h->nz)));
   HIP_TRY(hipDeviceSynchronize());
   h->any_wall = false;
   h->bbm_dirty = true;
@@ -483,18 +607,25 @@ int hlbm_add_obstacles(hlbm_domain* h, const hlbm_obstacle* list, int32_t n) {
   if (!(h->d.dx > 0.0)) return fail(HLBM_ERR_ARG, "desc.dx must be > 0 to rasterize obstacles");
   const double dx = h->d.dx;
   const int nn[3] = {h->nx, h->ny, h->nz};
+  const int per[3] = {h->d.periodic[0] ? 1 : 0, h->d.periodic[1] ? 1 : 0, h->d.periodic[2] ? 1 : 0};
   std::vector<ObstacleDev> obs;
   for (int k = 0; k < n; k++) {
     if (list[k].kind < 0 || list[k].kind > 2)
       return fail(HLBM_ERR_ARG, "obstacle kind must be 0 (ball), 1 (wall) or 2 (quadric)");
     ObstacleDev od;
-    if (obstacle_to_dev(list[k], nn, dx, od)) obs.push_back(od);
+    if (obstacle_to_dev(list[k], nn, dx, od, per)) obs.push_back(od);
   }
   if (obs.empty()) return HLBM_OK;
+  {
+    const int rc = materialize_for_change(h);
+    if (rc) return rc;
+  }
   ObstacleDev* dobs = nullptr;
   HIP_TRY(hipMalloc(&dobs, obs.size() * sizeof(ObstacleDev)));
   hipError_t e = hipMemcpy(dobs, obs.data(), obs.size() * sizeof(ObstacleDev), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = launch_rasterize(h->flags, h->nx, h->ny, h->nz, dx, dobs, (int)obs.size(), 0);
+  if (e == hipSuccess && h->gflags)
+    e = launch_rasterize_ghost(h->gflags, h->nx, h->ny, h->nz, per, dx, dobs, (int)obs.size(), 0);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   (void)hipFree(dobs);
   if (e != hipSuccess) return fail(HLBM_ERR_HIP, std::string("rasterize: ") + hipGetErrorString(e));
@@ -511,6 +642,10 @@ static size_t ref_count(const hlbm_domain* h) { return (size_t)(h->nx + 4) * (h-
 
 int hlbm_set_flags(hlbm_domain* h, const int32_t* obst, int32_t layout) {
   if (!h || !obst) return fail(HLBM_ERR_ARG, "NULL argument");
+  {
+    const int rc = materialize_for_change(h);
+    if (rc) return rc;
+  }
   std::vector<uint8_t> fl(h->N);
   size_t k = 0;
   for (int z = 0; z < h->nz; z++)
@@ -521,6 +656,26 @@ int hlbm_set_flags(hlbm_domain* h, const int32_t* obst, int32_t layout) {
         fl[k] = v == -2 ? 1 : 0;
       }
   HIP_TRY(hipMemcpy(h->flags, fl.data(), h->N, hipMemcpyHostToDevice));
+  if (h->gflags) {
+    const int per[3] = {h->d.periodic[0] ? 1 : 0, h->d.periodic[1] ? 1 : 0, h->d.periodic[2] ? 1 : 0};
+    if (layout == HLBM_LAYOUT_REF) {
+      // the first ghost layer as given (the reference's obstacle field over the All box)
+      std::vector<uint8_t> g(ghost_count(h->nx, h->ny, h->nz), 0);
+      size_t j = 0;
+      for (int z = -1; z <= h->nz; z++)
+        for (int y = -1; y <= h->ny; y++)
+          for (int x = -1; x <= h->nx; x++, j++) {
+            const int32_t v = obst[ref_index(h, x, y, z)];
+            if (v != -1 && v != -2) return fail(HLBM_ERR_ARG, "obstacle values must be -1 (fluid) or -2 (wall)");
+            g[j] = v == -2 ? 1 : 0;
+          }
+      HIP_TRY(hipMemcpy(h->gflags, g.data(), g.size(), hipMemcpyHostToDevice));
+    } else {
+      // node layout carries no ghosts: they take their periodic images' flags
+      HIP_TRY(launch_ghost_images(h->gflags, h->flags, h->nx, h->ny, h->nz, per, 0));
+      HIP_TRY(hipDeviceSynchronize());
+    }
+  }
   h->any_wall = std::any_of(fl.begin(), fl.end(), [](uint8_t v) { return v != 0; });
   h->bbm_dirty = true;
   return HLBM_OK;
@@ -858,6 +1013,7 @@ static int time_shape(hlbm_domain* h, void* scratch, int reps, hipStream_t s, hi
   StepParams p = base_params(h);
   p.write_macro = 0;
   p.skip_program = 0;
+  p.spec_val = h->spec_scratch;  // the listed previous-step values belong to the real state
   auto one = [&](int k) -> hipError_t {
     int mode;
     if (aa) {

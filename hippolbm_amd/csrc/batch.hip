@@ -334,6 +334,23 @@ int hlbm_batch_submit(hlbm_batch* b, const hlbm_solve_desc* s, int64_t* id) {
   if ((s->cavity_on[0] || s->cavity_on[1]) && s->periodic[2])
     return bfail(HLBM_ERR_ARG, "a moving wall needs a closed z axis");
   if (s->n_obstacles > 0 && (!s->obstacles || !(s->dx > 0.0))) return bfail(HLBM_ERR_ARG, "obstacles need dx > 0");
+  // hippoLBM bounces walls on a periodic face off ghost nodes that hold the
+  // previous step's populations (bounce_back.hpp:163-185, streaming.cu:107-111);
+  // single domains reproduce that (special_kernel), the batch kernel does not:
+  // obstacles whose candidate box reaches a periodic face are refused
+  if (s->wall_bb && s->n_obstacles > 0) {
+    for (int k = 0; k < s->n_obstacles; k++) {
+      const hlbm_obstacle& o = s->obstacles[k];
+      for (int d = 0; d < 3; d++) {
+        if (!s->periodic[d]) continue;
+        const double lo = o.kind == 0 ? o.p0[d] - o.radius : o.p0[d];
+        const double hi = o.kind == 0 ? o.p0[d] + o.radius : o.p1[d];
+        if (o.kind == 2 || int(lo / s->dx) <= 0 || int(hi / s->dx) >= s->nodes[d] - 1)
+          return bfail(HLBM_ERR_UNSUPPORTED,
+                       "batch solves take no wall_bounce_back obstacle that reaches a periodic face; use a domain");
+      }
+    }
+  }
   const uint64_t N = (uint64_t)s->nodes[0] * s->nodes[1] * s->nodes[2];
   const uint64_t stride = round_up(N, 128);
   if (stride * 19 > b->arena_elems || N > b->max_nodes) return bfail(HLBM_ERR_NOMEM, "solve can never fit the budget");

@@ -47,6 +47,67 @@ inline unsigned boundary_blocks(const StepParams& p) {
   return (unsigned)((n + BOUNDARY_BLOCK - 1) / BOUNDARY_BLOCK);
 }
 
+// One step: outflow capture (in-place AA), program-plane kernel, bulk kernel,
+// special_kernel (nodes next to a wall across a periodic face).  The
+// program-plane nodes, the special nodes and the bulk touch disjoint nodes
+// (A/B) or disjoint slots (AA), so their order is free.
+template <class S, int COLL, int MODE, int NPT, bool WIDE>
+hipError_t step_go(dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
+  hipError_t e = hipSuccess;
+  if constexpr (MODE == MODE_AA_EVEN || MODE == MODE_AA_ODD) {
+    if (p.oflow && !p.skip_program) {  // outflow values first: the step below updates nodes in place
+      hipLaunchKernelGGL((outflow_capture_kernel<S, MODE>), dim3((p.oflow_n + 255) / 256), dim3(256), 0, s, p);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+  }
+  const bool planes = MODE != MODE_COLLIDE && p.prog_planes;
+  if (planes && p.boundary_first) {
+    hipLaunchKernelGGL((boundary_kernel<S, COLL, MODE>), dim3(boundary_blocks(p)), dim3(BOUNDARY_BLOCK), 0, s, p);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (MODE != MODE_AA_FIX) {  // AA_FIX only touches program-plane nodes
+    hipLaunchKernelGGL((step_kernel<S, COLL, MODE, NPT, WIDE>), grid, block, 0, s, p);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (p.nspec) {
+      hipLaunchKernelGGL((special_kernel<S, COLL, MODE>), dim3((p.nspec + 63) / 64), dim3(64), 0, s, p);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+  }
+  if (!planes || p.boundary_first) return e;
+  hipLaunchKernelGGL((boundary_kernel<S, COLL, MODE>), dim3(boundary_blocks(p)), dim3(BOUNDARY_BLOCK), 0, s, p);
+  return hipGetLastError();
+}
+
+template <class S, int COLL, int MODE>
+hipError_t step_go_npt(int npt, dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
+  if (wide_offsets(p, sizeof(S))) return step_go<S, COLL, MODE, 1, true>(grid, block, p, s);
+  return npt == 2 ? step_go<S, COLL, MODE, 2, false>(grid, block, p, s)
+                  : step_go<S, COLL, MODE, 1, false>(grid, block, p, s);
+}
+
+template <class S>
+hipError_t step_dispatch(int coll, int mode, int npt, dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
+  if (mode == MODE_STREAM) return step_go_npt<S, COLL_BGK, MODE_STREAM>(npt, grid, block, p, s);  // no collision
+  if (mode == MODE_AA_STREAM) return step_go_npt<S, COLL_BGK, MODE_AA_STREAM>(npt, grid, block, p, s);
+  if (mode == MODE_AA_FIX) return step_go_npt<S, COLL_BGK, MODE_AA_FIX>(npt, grid, block, p, s);
+  if (mode == MODE_AA_EVEN)
+    return coll == COLL_BGK ? step_go_npt<S, COLL_BGK, MODE_AA_EVEN>(npt, grid, block, p, s)
+                            : step_go_npt<S, COLL_MRT, MODE_AA_EVEN>(npt, grid, block, p, s);
+  if (mode == MODE_AA_ODD)
+    return coll == COLL_BGK ? step_go_npt<S, COLL_BGK, MODE_AA_ODD>(npt, grid, block, p, s)
+                            : step_go_npt<S, COLL_MRT, MODE_AA_ODD>(npt, grid, block, p, s);
+  if (coll == COLL_BGK) {
+    if (mode == MODE_FUSED) return step_go_npt<S, COLL_BGK, MODE_FUSED>(npt, grid, block, p, s);
+    return step_go_npt<S, COLL_BGK, MODE_COLLIDE>(npt, grid, block, p, s);
+  }
+  if (mode == MODE_FUSED) return step_go_npt<S, COLL_MRT, MODE_FUSED>(npt, grid, block, p, s);
+  return step_go_npt<S, COLL_MRT, MODE_COLLIDE>(npt, grid, block, p, s);
+}
+
 // FP64 storage, FP64 compute, no contraction (step_f64.hip)
 hipError_t launch_step_f64(int coll, int mode, int npt, dim3 grid, dim3 block, const StepParams& p, hipStream_t s);
 // FP32 compute on shifted FP32/FP16/BF16 storage (step_lowp.hip); storage = hlbm_storage

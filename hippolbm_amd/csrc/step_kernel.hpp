@@ -11,7 +11,9 @@
 //          pre_bounce_back/post_bounce_back re-impose, bounce_back.hpp:92-156),
 //          plus the moving-wall term of cavity_z_0 / cavity_z_l (cavity.hpp:57-87),
 //        - obstacle cells by half-way bounce-back f_i(x) <- f*_opp(i)(x) when the
-//          upstream node is a wall (wall_bounce_back, bounce_back.hpp:163-185);
+//          upstream node is a wall (wall_bounce_back, bounce_back.hpp:163-185),
+//          except across a periodic face, where special_kernel reproduces the
+//          reference's bounce-back off a stale ghost node;
 //   2. runs the post-stream boundary program (post_bounce_back re-application,
 //      lid_driven_cavity, neumann) in the operator-list order of the deck;
 //   3. computes rho and u (macro_variables, macro_variables.hpp:48-79);
@@ -103,6 +105,11 @@ struct StepParams {
   double ef[Q];         // c_i . F, evaluated in the reference's order on the host
   double mrt[10];       // s1, s2, s4, s9, s10, s13, s16, weps, wepsj, wxx (mrt_custom)
   ProgEntry prog[MAX_PROG];
+  // nodes next to a wall across a periodic face (wall_bounce_back): special_kernel
+  uint32_t nspec;              // listed nodes
+  const uint32_t* spec_idx;    // node index
+  const uint32_t* spec_mask;   // bit i: f_i takes the node's own f_opp(i) of the previous step
+  void* spec_val;              // those values (C type), [i * nspec + k]
   int32_t boundary_first;      // launch the program-plane kernel before the bulk kernel
   int32_t nset;                // start_iteration set_distribution boxes
   int32_t set_box[MAX_SET][6]; // node box, inclusive: x0 y0 z0 x1 y1 z1
@@ -1233,6 +1240,7 @@ __global__ void __launch_bounds__(256) step_kernel(const StepParams p) {
         if (p.wall_bb && p.bbm) {  // the host never launches this variant with wall_bb and no mask
           wbits[k] = p.bbm[idx];
           wall[k] = (wbits[k] & 1u) != 0u;
+          if (wbits[k] >> 31) act[k] = false;  // special_kernel's node
         } else {
           wall[k] = p.flags[idx] != 0;
         }
@@ -1262,6 +1270,7 @@ __global__ void __launch_bounds__(256) step_kernel(const StepParams p) {
     if (act[k]) {
       wall[k] = p.flags[idx[k]] != 0;
       if (mode_skips_walls(MODE) && wall[k] && p.wall_bb) act[k] = false;
+      if (p.nspec && (p.bbm[idx[k]] >> 31)) act[k] = false;  // special_kernel's node
     }
     if (act[k]) gather_node<S, MODE>(p, x, y, z, idx[k], wall[k], f[k]);
   }
@@ -1295,6 +1304,55 @@ __global__ void __launch_bounds__(256) boundary_kernel(const StepParams p) {
     }
     t -= cnt;
   }
+}
+
+// Nodes next to a wall across a periodic face (wall_bounce_back; the list and
+// its masks come from k_find_special).  hippoLBM bounces such a wall off its
+// ghost neighbour, which still holds the populations of the previous step's
+// streaming (ghosts are refreshed only inside streaming, streaming.cu:107-111;
+// bounce_back.hpp:163-185): the population f_i(x) arriving from the wall is
+// f_opp(i)(x) of x's previous pre-collision state, not of the current
+// post-collision one.  Each listed node keeps those values in p.spec_val,
+// written whenever a kernel forms the node's pre-collision state (collision
+// modes) and read by the next gather.  None of these nodes lies on a plane of
+// the boundary program (the host rejects that), and none is a wall.
+template <class S, int COLL, int MODE>
+__global__ void __launch_bounds__(64) special_kernel(const StepParams p) {
+  using C = typename StoreOps<S>::C;
+  const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= p.nspec) return;
+  const uint32_t idx = p.spec_idx[k];
+  const uint32_t mask = p.spec_mask[k];
+  const uint32_t nxy = (uint32_t)p.nx * (uint32_t)p.ny;
+  const int z = (int)(idx / nxy);
+  const uint32_t r = idx - (uint32_t)z * nxy;
+  const int y = (int)(r / (uint32_t)p.nx), x = (int)(r % (uint32_t)p.nx);
+  C* sv = (C*)p.spec_val;
+  const size_t ns = p.nspec;
+  C f[Q];
+  gather_node<S, MODE>(p, x, y, z, idx, false, f);
+  if constexpr (!mode_reads_own(MODE)) {
+    static_for<1, Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      if ((mask >> i) & 1u) f[i] = sv[(size_t)i * ns + k];
+    });
+  }
+  if constexpr (mode_collides(MODE)) {
+    static_for<1, Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      if ((mask >> i) & 1u) sv[(size_t)i * ns + k] = f[OPP(i)];
+    });
+  }
+  collide_node<S, COLL, MODE>(p, idx, false, f);
+  if constexpr (MODE == MODE_AA_ODD) {
+    // direction opp(i) runs into the wall and bounces into slot i of the node,
+    // which the next (even) step reads as f_i: the pre-collision f_opp(i)
+    static_for<1, Q>([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      if ((mask >> i) & 1u) f[OPP(i)] = sv[(size_t)i * ns + k];
+    });
+  }
+  store_node<S, MODE>(p, x, y, z, idx, false, f);
 }
 
 // Outflow capture for an in-place AA step: before any node of the step is

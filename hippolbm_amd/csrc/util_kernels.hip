@@ -17,6 +17,20 @@ namespace hlbm {
 // obstacle.hpp:13-17,61-64,97,136-138.  Position of a node
 // is (local + offset) * dx = node * dx (grid.hpp:355-361).
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ bool obstacle_solid(const ObstacleDev& o, double px, double py, double pz) {
+  if (o.kind == 0) {
+    const double rx = px - o.p0[0], ry = py - o.p0[1], rz = pz - o.p0[2];
+    return (rx * rx + ry * ry + rz * rz) <= o.r2;
+  }
+  if (o.kind == 2) {
+    // Quadric::solid: [p,1] Q [p,1]^T <= 0, as p . (Q [p,1]) with the rows of Q first
+    double r[4];
+    for (int i = 0; i < 4; i++) r[i] = o.q[4 * i] * px + o.q[4 * i + 1] * py + o.q[4 * i + 2] * pz + o.q[4 * i + 3];
+    return px * r[0] + py * r[1] + pz * r[2] + r[3] <= 0.0;
+  }
+  return o.p0[0] < px && px < o.p1[0] && o.p0[1] < py && py < o.p1[1] && o.p0[2] < pz && pz < o.p1[2];
+}
+
 __global__ void k_rasterize(uint8_t* flags, int nx, int ny, int nz, double dx, const ObstacleDev* obs, int nobs) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
@@ -31,20 +45,127 @@ __global__ void k_rasterize(uint8_t* flags, int nx, int ny, int nz, double dx, c
   for (int k = 0; k < nobs; k++) {
     const ObstacleDev o = obs[k];
     if (x < o.lo[0] || x > o.hi[0] || y < o.lo[1] || y > o.hi[1] || z < o.lo[2] || z > o.hi[2]) continue;
-    if (o.kind == 0) {
-      const double rx = px - o.p0[0], ry = py - o.p0[1], rz = pz - o.p0[2];
-      if ((rx * rx + ry * ry + rz * rz) <= o.r2) solid = true;
-    } else if (o.kind == 2) {
-      // Quadric::solid: [p,1] Q [p,1]^T <= 0, as p . (Q [p,1]) with the rows of Q first
-      double r[4];
-      for (int i = 0; i < 4; i++) r[i] = o.q[4 * i] * px + o.q[4 * i + 1] * py + o.q[4 * i + 2] * pz + o.q[4 * i + 3];
-      if (px * r[0] + py * r[1] + pz * r[2] + r[3] <= 0.0) solid = true;
-    } else {
-      if (o.p0[0] < px && px < o.p1[0] && o.p0[1] < py && py < o.p1[1] && o.p0[2] < pz && pz < o.p1[2])
-        solid = true;
-    }
+    if (obstacle_solid(o, px, py, pz)) solid = true;
   }
   if (solid) flags[idx] = 1;
+}
+
+// Ghost nodes of the periodic axes, first layer (coordinate -1 or n on the
+// axis): the reference rasterizes them at their own positions, node * dx with
+// a negative node index below the grid (update_obstacles.cu:58-67 over the
+// Extend box), not as periodic images.
+__device__ __forceinline__ size_t ghost_index(int x, int y, int z, int nx, int ny) {
+  return ((size_t)(z + 1) * (size_t)(ny + 2) + (size_t)(y + 1)) * (size_t)(nx + 2) + (size_t)(x + 1);
+}
+
+// Coordinates of padded-grid thread (tx, ty, tz) when it is a ghost node of the
+// periodic axes' first layer; false for Real nodes and for positions outside a
+// non-periodic face (not in the Extend box).
+__device__ __forceinline__ bool ghost_node(int tx, int ty, int tz, int nx, int ny, int nz, int px, int py, int pz,
+                                           int& x, int& y, int& z) {
+  x = tx - 1;
+  y = ty - 1;
+  z = tz - 1;
+  if (x > nx || y > ny || z > nz) return false;
+  const bool ox = x < 0 || x >= nx, oy = y < 0 || y >= ny, oz = z < 0 || z >= nz;
+  if (!(ox || oy || oz)) return false;
+  return !((ox && !px) || (oy && !py) || (oz && !pz));
+}
+
+__global__ void k_rasterize_ghost(uint8_t* gflags, int nx, int ny, int nz, int pxx, int pyy, int pzz, double dx,
+                                  const ObstacleDev* obs, int nobs) {
+  int x, y, z;
+  if (!ghost_node(blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y, blockIdx.z, nx, ny,
+                  nz, pxx, pyy, pzz, x, y, z))
+    return;
+  const double px = (double(x + 2) + double(-2)) * dx;
+  const double py = (double(y + 2) + double(-2)) * dx;
+  const double pz = (double(z + 2) + double(-2)) * dx;
+  bool solid = false;
+  for (int k = 0; k < nobs; k++) {
+    const ObstacleDev o = obs[k];
+    if (x < o.glo[0] || x > o.ghi[0] || y < o.glo[1] || y > o.ghi[1] || z < o.glo[2] || z > o.ghi[2]) continue;
+    if (obstacle_solid(o, px, py, pz)) solid = true;
+  }
+  if (solid) gflags[ghost_index(x, y, z, nx, ny)] = 1;
+}
+
+__global__ void k_ghost_images(uint8_t* gflags, const uint8_t* flags, int nx, int ny, int nz, int px, int py, int pz) {
+  int x, y, z;
+  if (!ghost_node(blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y, blockIdx.z, nx, ny,
+                  nz, px, py, pz, x, y, z))
+    return;
+  const int ix = x < 0 ? nx - 1 : (x >= nx ? 0 : x);
+  const int iy = y < 0 ? ny - 1 : (y >= ny ? 0 : y);
+  const int iz = z < 0 ? nz - 1 : (z >= nz ? 0 : z);
+  gflags[ghost_index(x, y, z, nx, ny)] = flags[((size_t)iz * ny + iy) * nx + ix];
+}
+
+// ---------------------------------------------------------------------------
+// Walls across a periodic face (wall_bounce_back, bounce_back.hpp:163-185).
+// The reference bounces a wall node w off its neighbour w + c_i before
+// streaming, over the Extend box.  When w lies on the face of a periodic axis
+// and w + c_i across it, that neighbour is a ghost node: its flag is the
+// ghost's own (rasterized at its position), and its populations are those it
+// received in the previous step's streaming, since ghosts are refreshed only
+// inside streaming (streaming.cu:107-111).  Those are the populations of the
+// ghost's periodic image x at the start of the step (x's pre-collision state)
+// as long as no boundary operator changes x.  So for a fluid node x whose
+// upstream node x - c_i lies across periodic faces only and wraps to a wall:
+//   - ghost of x (x shifted across the face) fluid: f_i(x) receives f_opp(i)(x)
+//     of x's previous pre-collision state: bit i of the node's mask;
+//   - ghost of x a wall: f_i(x) receives the wall's own streamed value, which
+//     the engine does not track (walls hold no populations it follows): bit 31,
+//     the host rejects the configuration.
+// Nodes with a marked direction are appended to (out_idx, out_mask) and get
+// bit 31 in the bulk mask, so the bulk kernels leave them to special_kernel.
+// ---------------------------------------------------------------------------
+__global__ void k_find_special(const uint8_t* flags, const uint8_t* gflags, int nx, int ny, int nz, int px, int py,
+                               int pz, uint32_t* bbm, uint32_t* out_idx, uint32_t* out_mask, uint32_t* count,
+                               uint32_t cap) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int y = blockIdx.y * blockDim.y + threadIdx.y;
+  const int z = blockIdx.z;
+  if (x >= nx || y >= ny) return;
+  const bool face = (px && (x == 0 || x == nx - 1)) || (py && (y == 0 || y == ny - 1)) ||
+                    (pz && (z == 0 || z == nz - 1));
+  if (!face) return;
+  const size_t idx = ((size_t)z * ny + y) * nx + x;
+  if (flags[idx]) return;
+  uint32_t m = 0;
+  static_for<1, Q>([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    int sx = x - CX(i), sy = y - CY(i), sz = z - CZ(i);
+    int gx = x, gy = y, gz = z;  // the ghost of x across the crossed faces
+    bool crossed = false;
+    if (sx < 0 || sx >= nx) {
+      if (!px) return;
+      gx = sx < 0 ? x + nx : x - nx;
+      sx = sx < 0 ? nx - 1 : 0;
+      crossed = true;
+    }
+    if (sy < 0 || sy >= ny) {
+      if (!py) return;
+      gy = sy < 0 ? y + ny : y - ny;
+      sy = sy < 0 ? ny - 1 : 0;
+      crossed = true;
+    }
+    if (sz < 0 || sz >= nz) {
+      if (!pz) return;
+      gz = sz < 0 ? z + nz : z - nz;
+      sz = sz < 0 ? nz - 1 : 0;
+      crossed = true;
+    }
+    if (!crossed || !flags[((size_t)sz * ny + sy) * nx + sx]) return;
+    m |= gflags[ghost_index(gx, gy, gz, nx, ny)] ? (1u << 31) : (1u << i);
+  });
+  if (!m) return;
+  const uint32_t k = atomicAdd(count, 1u);
+  if (k < cap) {
+    out_idx[k] = (uint32_t)idx;
+    out_mask[k] = m;
+  }
+  bbm[idx] |= 1u << 31;
 }
 
 // set_distribution (set_distribution.hpp:38-41): f_i = value * w_i on a node box.
@@ -418,6 +539,31 @@ hipError_t launch_rasterize(uint8_t* flags, int nx, int ny, int nz, double dx, c
                             hipStream_t s) {
   dim3 b(64, 4, 1), g((nx + 63) / 64, (ny + 3) / 4, nz);
   hipLaunchKernelGGL(k_rasterize, g, b, 0, s, flags, nx, ny, nz, dx, obs, nobs);
+  return hipGetLastError();
+}
+
+static dim3 padded_grid(int nx, int ny, int nz) { return dim3((nx + 2 + 63) / 64, (ny + 2 + 3) / 4, nz + 2); }
+
+hipError_t launch_rasterize_ghost(uint8_t* gflags, int nx, int ny, int nz, const int* per, double dx,
+                                  const ObstacleDev* obs, int nobs, hipStream_t s) {
+  hipLaunchKernelGGL(k_rasterize_ghost, padded_grid(nx, ny, nz), dim3(64, 4, 1), 0, s, gflags, nx, ny, nz, per[0],
+                     per[1], per[2], dx, obs, nobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_ghost_images(uint8_t* gflags, const uint8_t* flags, int nx, int ny, int nz, const int* per,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_ghost_images, padded_grid(nx, ny, nz), dim3(64, 4, 1), 0, s, gflags, flags, nx, ny, nz, per[0],
+                     per[1], per[2]);
+  return hipGetLastError();
+}
+
+hipError_t launch_find_special(const uint8_t* flags, const uint8_t* gflags, int nx, int ny, int nz, const int* per,
+                               uint32_t* bbm, uint32_t* out_idx, uint32_t* out_mask, uint32_t* count, uint32_t cap,
+                               hipStream_t s) {
+  dim3 b(64, 4, 1), g((nx + 63) / 64, (ny + 3) / 4, nz);
+  hipLaunchKernelGGL(k_find_special, g, b, 0, s, flags, gflags, nx, ny, nz, per[0], per[1], per[2], bbm, out_idx,
+                     out_mask, count, cap);
   return hipGetLastError();
 }
 

@@ -160,9 +160,17 @@ int hlbm_set_collision(hlbm_domain* h, int32_t collision, const double* mrt_rate
 int hlbm_init_obst(hlbm_domain* h);
 /* update_obstacles for a list of balls/walls/quadrics in obstacle-id order,
  * rasterized on the device with the reference's rules
- * (src/obstacle/update_obstacles.cu:32-71, obstacle.hpp:13-17,61-64,97,136-138). */
+ * (src/obstacle/update_obstacles.cu:32-71, obstacle.hpp:13-17,61-64,97,136-138),
+ * including the first ghost layer of the periodic axes at its own positions:
+ * with wall_bounce_back, a wall across a periodic face bounces off that ghost
+ * as in the reference (src/bcs/include/hippoLBM/bcs/bounce_back.hpp:163-185).
+ * A wall ghost over a fluid periodic image next to a wall node makes
+ * hlbm_step fail with HLBM_ERR_UNSUPPORTED. */
 int hlbm_add_obstacles(hlbm_domain* h, const hlbm_obstacle* list, int32_t n);
-/* Obstacle field as int32 -1 (fluid) / -2 (wall), node or reference layout. */
+/* Obstacle field as int32 -1 (fluid) / -2 (wall), node or reference layout.
+ * set, reference layout: the first ghost layer of the periodic axes is taken
+ * as given; node layout: those ghosts take their periodic images' flags.
+ * get writes the Real nodes only. */
 int hlbm_set_flags(hlbm_domain* h, const int32_t* obst, int32_t layout);
 int hlbm_get_flags(hlbm_domain* h, int32_t* obst, int32_t layout);
 /* set_distribution: f_i = value*w_i everywhere (bounds == NULL) or on the
