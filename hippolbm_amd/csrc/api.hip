@@ -187,15 +187,22 @@ static int build_special(hlbm_domain* h, hipStream_t s) {
   return HLBM_OK;
 }
 
-// The previous-step values stand for the ghost's populations only while no
-// boundary operator treats the node differently from its ghost: the ghost
-// lies outside the planes of the crossed axes, the reference's plane operators
-// (pre/post_bounce_back, cavity, lid, neumann) and set_distribution boxes act
-// on ghosts by their own positions and with their own stale inputs.  Those
-// combinations, and a wall ghost over a fluid image, are refused.
+// The previous-step values stand for the ghost's populations only while the
+// operators between streaming and the next collision treat the node and its
+// ghost alike.  Plane operators that are functions of the node's own
+// populations (lid_driven_cavity, neumann, pressure_inlet) do, on planes of
+// non-periodic axes, which hold the ghost too (plane lists span the All box,
+// grid_region.hpp:148-193).  Refused: post_bounce_back faces (the ghost
+// restores its own stale saved values, bounce_back.hpp:115-140), program planes
+// of periodic axes (the ghost across the face is not on them), start_iteration
+// boxes (applied to the ghost by its own position), an outflow reading the node
+// as its first interior layer, and a wall ghost over a fluid image.
 static int check_special(hlbm_domain* h) {
   if (!h->nspec || h->spec_checked == h->plan_serial) return HLBM_OK;
   const StepParams& P = h->plan;
+  int closed = 0, open = 0;  // planes of the non-periodic / periodic axes
+  for (int pl = 0; pl < 6; pl++) (h->d.periodic[plane_dim(pl)] ? open : closed) |= 1 << pl;
+  const int n[3] = {h->nx, h->ny, h->nz};
   for (size_t k = 0; k < h->spec_host_idx.size(); k++) {
     if (h->spec_host_mask[k] >> 31)
       return fail(HLBM_ERR_UNSUPPORTED,
@@ -203,17 +210,24 @@ static int check_special(hlbm_domain* h) {
                   "position, is a wall while its periodic image is fluid (hippoLBM then streams the wall's own "
                   "populations, which this engine does not keep)");
     const uint32_t idx = h->spec_host_idx[k];
-    const int x = (int)(idx % (uint32_t)h->nx), y = (int)(idx / (uint32_t)h->nx % (uint32_t)h->ny),
-              z = (int)(idx / ((uint32_t)h->nx * (uint32_t)h->ny));
-    const int planes = (z == 0 ? 1 : 0) | (z == h->nz - 1 ? 2 : 0) | (y == 0 ? 4 : 0) | (y == h->ny - 1 ? 8 : 0) |
-                       (x == 0 ? 16 : 0) | (x == h->nx - 1 ? 32 : 0);
-    int closed = 0;  // planes of the non-periodic axes
-    for (int pl = 0; pl < 6; pl++)
-      if (!h->d.periodic[plane_dim(pl)]) closed |= 1 << pl;
-    if ((planes & P.prog_planes) || (P.plane_bb && (planes & closed)) || P.nset)
+    const int c[3] = {(int)(idx % (uint32_t)h->nx), (int)(idx / (uint32_t)h->nx % (uint32_t)h->ny),
+                      (int)(idx / ((uint32_t)h->nx * (uint32_t)h->ny))};
+    int planes = 0;
+    for (int pl = 0; pl < 6; pl++) {
+      const int d = plane_dim(pl);
+      if (c[d] == (plane_side(pl) ? n[d] - 1 : 0)) planes |= 1 << pl;
+    }
+    bool outflow_reads = false;
+    for (int j = 0; j < P.nprog; j++) {
+      if (P.prog[j].kind != PK_OUTFLOW) continue;
+      const int d = plane_dim(P.prog[j].plane);
+      outflow_reads |= c[d] == (plane_side(P.prog[j].plane) ? n[d] - 2 : 1);
+    }
+    if ((P.plane_bb && (planes & closed)) || (planes & P.prog_planes & open) || P.nset || outflow_reads)
       return fail(HLBM_ERR_UNSUPPORTED,
-                  "wall_bounce_back: a node next to a wall across a periodic face lies on a plane of a boundary "
-                  "operator or under start_iteration set_distribution");
+                  "wall_bounce_back: a node next to a wall across a periodic face lies on a pre/post_bounce_back "
+                  "face, on a boundary-operator plane of a periodic axis, under start_iteration set_distribution "
+                  "or next to an outflow plane");
   }
   h->spec_checked = h->plan_serial;
   return HLBM_OK;

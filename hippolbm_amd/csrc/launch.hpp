@@ -71,11 +71,11 @@ hipError_t step_go(dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
     hipLaunchKernelGGL((step_kernel<S, COLL, MODE, NPT, WIDE>), grid, block, 0, s, p);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (p.nspec) {
-      hipLaunchKernelGGL((special_kernel<S, COLL, MODE>), dim3((p.nspec + 63) / 64), dim3(64), 0, s, p);
-      e = hipGetLastError();
-      if (e != hipSuccess) return e;
-    }
+  }
+  if (p.nspec) {  // special nodes on program planes are left out of the program-plane kernel too
+    hipLaunchKernelGGL((special_kernel<S, COLL, MODE>), dim3((p.nspec + 63) / 64), dim3(64), 0, s, p);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
   }
   if (!planes || p.boundary_first) return e;
   hipLaunchKernelGGL((boundary_kernel<S, COLL, MODE>), dim3(boundary_blocks(p)), dim3(BOUNDARY_BLOCK), 0, s, p);

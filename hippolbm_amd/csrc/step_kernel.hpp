@@ -1193,6 +1193,7 @@ __device__ __forceinline__ void process_node(const StepParams& p, int x, int y, 
   const uint32_t idx = ((uint32_t)z * (uint32_t)p.ny + (uint32_t)y) * (uint32_t)p.nx + (uint32_t)x;
   const bool wall = p.flags[idx] != 0;
   if (mode_skips_walls(MODE) && wall && p.wall_bb) return;
+  if (p.nspec && (p.bbm[idx] >> 31)) return;  // special_kernel's node
   C f[Q];
   gather_node<S, MODE>(p, x, y, z, idx, wall, f);
   if constexpr (PROG && MODE != MODE_COLLIDE) {
@@ -1314,8 +1315,11 @@ __global__ void __launch_bounds__(256) boundary_kernel(const StepParams p) {
 // f_opp(i)(x) of x's previous pre-collision state, not of the current
 // post-collision one.  Each listed node keeps those values in p.spec_val,
 // written whenever a kernel forms the node's pre-collision state (collision
-// modes) and read by the next gather.  None of these nodes lies on a plane of
-// the boundary program (the host rejects that), and none is a wall.
+// modes, after the boundary program: the ghost gets the same plane operators
+// from the same inputs) and read by the next gather.  None of these nodes is a
+// wall; the host refuses the operators that treat a node and its ghost
+// differently (pre/post_bounce_back faces, programs on planes of periodic axes,
+// start_iteration boxes, outflow reading the node).
 template <class S, int COLL, int MODE>
 __global__ void __launch_bounds__(64) special_kernel(const StepParams p) {
   using C = typename StoreOps<S>::C;
@@ -1336,6 +1340,10 @@ __global__ void __launch_bounds__(64) special_kernel(const StepParams p) {
       constexpr int i = decltype(ic)::value;
       if ((mask >> i) & 1u) f[i] = sv[(size_t)i * ns + k];
     });
+  }
+  if constexpr (MODE != MODE_COLLIDE) {
+    if (!p.skip_program && (node_planes(x, y, z, p.nx, p.ny, p.nz) & p.prog_planes))
+      apply_program<S, MODE>(p, x, y, z, idx, false, f);
   }
   if constexpr (mode_collides(MODE)) {
     static_for<1, Q>([&](auto ic) {

@@ -10,7 +10,8 @@ the affected nodes.  Cases:
 
 * the shape of example/lbm_quadrics.msp: a y-cylinder across the periodic y
   faces, a sphere and a cone, Zou/He walls on z, a body force (ghost flags equal
-  the periodic images);
+  the periodic images); with a wider cone, nodes next to walls across the y
+  faces also lie on the Zou/He planes (as in the example itself);
 * a wall box from y = -0.5 to y = 2.5 with dx = 1: the strict AABB test leaves
   the ghost at y = -1 fluid while its image y = ny - 1 is fluid too, and the
   Real walls y = 0..2 border the periodic face;
@@ -77,6 +78,13 @@ boundary_conditions:
 pre_stream_bcs:
   - wall_bounce_back
 """
+
+# the y-cylinder moved down to cut the z = 0 face: nodes next to it across the
+# periodic y faces lie on the neumann plane (as the cone does in
+# example/lbm_quadrics.msp)
+QUADRICS_ON_PLANES = QUADRICS_ACROSS.replace("""        - scale: [ 0.04, 1, 0.04 ]
+        - translate: [ 0.1, 0.06, 0.1 ]""", """        - scale: [ 0.04, 1, 0.04 ]
+        - translate: [ 0.1, 0.06, 0.02 ]""")
 
 WALL_AT_FACE = """
 do_domain:
@@ -158,6 +166,22 @@ def test_quadrics_across_periodic_faces_bitwise(pattern, steps):
     o.macro_variables()
     fluid = o.real(o.obst()) == -1
     assert np.array_equal(rho[fluid], o.real(o.m0())[fluid])
+    sim.dom.close()
+
+
+@pytest.mark.parametrize("pattern", ["ab", "aa"])
+@pytest.mark.parametrize("steps", [20, 21])
+def test_walls_across_periodic_faces_on_neumann_planes_bitwise(pattern, steps):
+    say("quadrics across periodic y faces, on the neumann planes", pattern, steps)
+    assert QUADRICS_ON_PLANES != QUADRICS_ACROSS
+    sim, ref, flags = _run(QUADRICS_ON_PLANES, "quadrics_on_planes", pattern, steps)
+    assert (flags[0, 0, :] == -2).sum() > 0  # walls on the y face at z = 0
+    nbad, maxdiff = _fluid_bitwise(sim, ref)
+    assert nbad == 0, (nbad, maxdiff)
+    ref.step(3)
+    sim.run(3)
+    nbad, maxdiff = _fluid_bitwise(sim, ref)
+    assert nbad == 0, (nbad, maxdiff)
     sim.dom.close()
 
 
