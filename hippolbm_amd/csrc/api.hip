@@ -966,6 +966,18 @@ int hlbm_plane_profile(hlbm_domain* h, int32_t dim, double scale, double* sum, i
   return HLBM_OK;
 }
 
+int hlbm_plane_profile_noalloc(hlbm_domain* h, int32_t dim, double scale, hlbm_plane_stat* out, void* stream) {
+  if (!h || !out) return fail(HLBM_ERR_ARG, "NULL argument");
+  if (dim < 0 || dim > 2) return fail(HLBM_ERR_ARG, "dim must be 0 (x), 1 (y) or 2 (z)");
+  if (profile_parts(h->nx, h->ny, h->nz, dim) > (size_t)HLBM_PROFILE_SCRATCH_COLUMNS)
+    return fail(HLBM_ERR_ARG, "plane profile: more columns than HLBM_PROFILE_SCRATCH_COLUMNS");
+  ProfileStat* part = nullptr;
+  HIP_TRY(profile_scratch(&part));
+  HIP_TRY(launch_plane_profile(h->m1, h->flags, h->N, h->nx, h->ny, h->nz, dim, scale, part, (ProfileStat*)out,
+                               (hipStream_t)stream));
+  return HLBM_OK;
+}
+
 int hlbm_line_velocity(hlbm_domain* h, const int32_t start[3], int32_t axis, int32_t n, double scale, double* u) {
   if (!h || !start || !u) return fail(HLBM_ERR_ARG, "NULL argument");
   if (axis < 0 || axis > 2 || n < 1) return fail(HLBM_ERR_ARG, "axis must be 0..2 and n >= 1");

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "step_kernel.hpp"
 #include "util_kernels.hpp"
 
@@ -664,6 +666,35 @@ hipError_t launch_plane_profile(const double* m1, const uint8_t* flags, uint32_t
   hipLaunchKernelGGL(k_profile_planes, dim3((unsigned)planes), dim3(256), 0, s, (const ProfileStat*)part, nx, na, dim,
                      out);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Column partials of plane_velocity_profile for hlbm_plane_profile_noalloc: a
+// file-scope device array, one copy per GPU for the life of the process.  In
+// host code its name is only a stand-in; kernels get the address that
+// hipGetSymbolAddress returns on the current device, looked up once per device.
+// k_profile_cols writes every record it covers before k_profile_planes reads
+// it, so nothing has to start at zero.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) ProfileStat scratch[HLBM_PROFILE_SCRATCH_COLUMNS];
+
+hipError_t profile_scratch(ProfileStat** out) {
+  constexpr int MAX_DEV = 64;
+  static std::mutex mu;
+  static ProfileStat* ptr[MAX_DEV] = {};
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= MAX_DEV) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> g(mu);
+  if (!ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    ptr[dev] = (ProfileStat*)p;
+  }
+  *out = ptr[dev];
+  return hipSuccess;
 }
 
 hipError_t launch_line_velocity(const double* m1, uint32_t N, int nx, int ny, int nz, const int* start, int axis,

@@ -3,6 +3,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 
 #include "../../include/hlbm.h"
@@ -102,6 +103,13 @@ hipError_t launch_reduce(const double* m0, const double* m1, const uint8_t* flag
 size_t profile_parts(int nx, int ny, int nz, int dim);
 hipError_t launch_plane_profile(const double* m1, const uint8_t* flags, uint32_t N, int nx, int ny, int nz, int dim,
                                 double scale, ProfileStat* part, ProfileStat* out, hipStream_t s);
+// The library-owned column partials of hlbm_plane_profile_noalloc on the
+// current device (HLBM_PROFILE_SCRATCH_COLUMNS records); no allocation.
+hipError_t profile_scratch(ProfileStat** out);
+static_assert(sizeof(ProfileStat) == sizeof(hlbm_plane_stat), "hlbm_plane_stat is ProfileStat's ABI form");
+static_assert(offsetof(ProfileStat, vmax) == offsetof(hlbm_plane_stat, vmax) &&
+                  offsetof(ProfileStat, count) == offsetof(hlbm_plane_stat, count),
+              "hlbm_plane_stat is ProfileStat's ABI form");
 hipError_t launch_line_velocity(const double* m1, uint32_t N, int nx, int ny, int nz, const int* start, int axis,
                                 int n, double scale, double* out, hipStream_t s);
 

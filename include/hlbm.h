@@ -222,6 +222,24 @@ int hlbm_reduce_stats(hlbm_domain* h, double out[7]);
  * order (deterministic); the reference adds with atomics. */
 int hlbm_plane_profile(hlbm_domain* h, int32_t dim, double scale, double* sum, int32_t* count, double* vmin,
                        double* vmax);
+/* The same statistics without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Asynchronous on `stream`; writes nodes[dim] records into the caller's DEVICE
+ * array `out`, bit for bit what hlbm_plane_profile returns.  The per-column
+ * partials live in a library-owned static device array of
+ * HLBM_PROFILE_SCRATCH_COLUMNS records (32 bytes each, 16 MiB), one per GPU for
+ * the life of the process: a domain with more than that many columns,
+ * nodes[0] * (dim == 1 ? nodes[1] : nodes[2]), is refused with HLBM_ERR_ARG.
+ * Every call on a GPU shares that array, so calls that overlap in time on one
+ * GPU (several streams or host threads) race; calls on one stream are ordered. */
+#define HLBM_PROFILE_SCRATCH_COLUMNS (1 << 19)
+typedef struct {
+  double sum, vmin, vmax;
+  int32_t count, pad;
+} hlbm_plane_stat;
+int hlbm_plane_profile_noalloc(hlbm_domain* h, int32_t dim, double scale, hlbm_plane_stat* out, void* stream);
+
 /* plot_line_velocity (src/check/plot_over_line.cu:77-90,163-172): scale*u of
  * the domain's u field at the n nodes start + k*e_axis (node coordinates,
  * 0-based), written u[3k + c]; nodes outside the domain give 0. */
