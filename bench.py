@@ -275,11 +275,7 @@ def copy_bandwidth():
     return 2 * nbytes / (ms.value * 1e-3) / 1e9
 
 
-def cpu_baseline(n=128, budget_s=12.0, max_steps=400):
-    """The FP64 oracle (a CPU restatement of the reference step chain, OpenMP
-    static) on the same Couette workload at n^3, timed on this host."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as orc
+def _oracle_couette(orc, n):
     o = orc.Oracle((n, n, n), (True, True, False), 1.0)
     o.init_obst()
     o.set_distribution(1.0)
@@ -292,16 +288,48 @@ def cpu_baseline(n=128, budget_s=12.0, max_steps=400):
         o.cavity(1, (0.05, 0.0, 0.0))
         o.streaming()
         o.post_bounce_back()
+    return step
 
-    step()
+
+def _time_sample(step, nodes, budget_s, max_steps, min_steps=2):
+    step()  # first touch of the lattice outside the timed region
     steps = 0
     t0 = time.perf_counter()
-    while steps < max_steps and time.perf_counter() - t0 < budget_s:
+    while steps < max_steps and (steps < min_steps or time.perf_counter() - t0 < budget_s):
         step()
         steps += 1
     dt = time.perf_counter() - t0
-    return {"value": n ** 3 * steps / dt / 1e6, "unit": "MLUPS", "cores": orc.lib().orc_threads(), "kind": "port",
-            "sample": f"oracle FP64 Couette {n}^3 BGK, {steps} steps ({dt:.1f} s), OpenMP static"}
+    return nodes * steps / dt / 1e6, steps, dt
+
+
+def cpu_baseline(n=512, budget_s=10.0):
+    """The FP64 oracle (a CPU restatement of the reference step chain, OpenMP
+    static, the reference's SoA layout with two ghost layers) timed on this host:
+    the headline figure on a bounded sample of the same Couette n^3 workload, on
+    all host cores; beside it the 64^3 Couette and the porous MRT channel
+    configurations on all cores, and the 64^3 Couette on one core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc
+    from hippolbm_amd import workloads
+    cores = orc.lib().orc_threads()
+    mlups, steps, dt = _time_sample(_oracle_couette(orc, n), n ** 3, budget_s, 400)
+    legs = {}
+    m, k, t = _time_sample(_oracle_couette(orc, 64), 64 ** 3, 3.0, 2000)
+    legs["couette_64^3_bgk"] = {"mlups": m, "steps": k, "s": t, "cores": cores}
+    run = orc.OracleRun(workloads.porous_channel())
+    m, k, t = _time_sample(lambda: run.step(1), 256 * 128 * 128, 5.0, 5000)
+    legs["porous_channel_256x128x128_mrt"] = {"mlups": m, "steps": k, "s": t, "cores": cores}
+    del run
+    orc.lib().orc_set_threads(1)
+    try:
+        m, k, t = _time_sample(_oracle_couette(orc, 64), 64 ** 3, 3.0, 2000)
+    finally:
+        orc.lib().orc_set_threads(cores)
+    legs["couette_64^3_bgk_1core"] = {"mlups": m, "steps": k, "s": t, "cores": 1}
+    return {"value": mlups, "unit": "MLUPS", "cores": cores, "kind": "port",
+            "sample": f"oracle FP64 Couette {n}^3 BGK (the same workload), {steps} steps ({dt:.1f} s), "
+                      "OpenMP static",
+            "legs": legs}
 
 
 def read_traffic(name):

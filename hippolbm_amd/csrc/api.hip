@@ -94,6 +94,11 @@ static dim3 grid_for(const hlbm_domain* h) {
   return dim3(gx * tiles_y(h) * h->nz, 1, 1);
 }
 
+// The rest state the domain's shifted storage is relative to (RestShift).
+static RestShift dom_shift(const hlbm_domain* h) {
+  return rest_shift(h->d.collision == HLBM_MRT, h->d.mrt_custom, h->d.mrt_rates);
+}
+
 static StepParams base_params(hlbm_domain* h) {
   StepParams p = h->plan;
   p.nx = h->nx;
@@ -110,6 +115,10 @@ static StepParams base_params(hlbm_domain* h) {
   // c_i . F in the reference's order: ex*Fx + ey*Fy + ez*Fz (bgk.hpp:57, mrt.hpp:188)
   for (int i = 0; i < Q; i++) p.ef[i] = CX(i) * h->d.fext[0] + CY(i) * h->d.fext[1] + CZ(i) * h->d.fext[2];
   set_fp32_constants(p);
+  {
+    const RestShift sh = dom_shift(h);
+    for (int k = 0; k < 3; k++) p.sig[k] = (float)sh.s[k];
+  }
   p.mrt_custom = h->d.mrt_custom;
   p.gx = (int)((h->nx + h->block.x - 1) / h->block.x);
   p.gy = (int)tiles_y(h);
@@ -391,11 +400,13 @@ int hlbm_domain_create(const hlbm_domain_desc* desc, hlbm_domain** out) {
     return cleanup("memset");
   if (hipDeviceSynchronize() != hipSuccess) return cleanup("sync");
   *out = h;
-  // zero populations in the storage's own encoding (shifted storage holds -w_i)
+  // zero populations in the storage's own encoding (shifted storage holds minus the rest state)
   if (h->d.storage != HLBM_F64) {
     int lo[3] = {0, 0, 0}, hi[3] = {h->nx - 1, h->ny - 1, h->nz - 1};
-    if (launch_set_distribution(h->d.storage, h->buf[0], h->stride, h->nx, h->ny, lo, hi, 0.0, 0) != hipSuccess ||
-        (!aa && launch_set_distribution(h->d.storage, h->buf[1], h->stride, h->nx, h->ny, lo, hi, 0.0, 0) != hipSuccess) ||
+    const RestShift sh = dom_shift(h);
+    if (launch_set_distribution(h->d.storage, h->buf[0], h->stride, h->nx, h->ny, lo, hi, 0.0, sh, 0) != hipSuccess ||
+        (!aa && launch_set_distribution(h->d.storage, h->buf[1], h->stride, h->nx, h->ny, lo, hi, 0.0, sh, 0) !=
+                    hipSuccess) ||
         hipDeviceSynchronize() != hipSuccess) {
       hlbm_domain_destroy(h);
       *out = nullptr;
@@ -727,7 +738,8 @@ int hlbm_set_distribution(hlbm_domain* h, double value, const double* bounds) {
       hi[d] = std::min(b, nn[d] - 1);
     }
   }
-  HIP_TRY(launch_set_distribution(h->d.storage, h->buf[h->cur], h->stride, h->nx, h->ny, lo, hi, value, 0));
+  HIP_TRY(launch_set_distribution(h->d.storage, h->buf[h->cur], h->stride, h->nx, h->ny, lo, hi, value, dom_shift(h),
+                                  0));
   HIP_TRY(hipDeviceSynchronize());
   return HLBM_OK;
 }
@@ -741,7 +753,8 @@ int hlbm_init_double_couette(hlbm_domain* h, int32_t dim, const double U[3]) {
   // dU = Uc / (0.5 * (domain_size[dim] - 1)) (init_double_couette.cu:85-107)
   double dU[3];
   for (int d = 0; d < 3; d++) dU[d] = U[d] / (0.5 * (n - 1));
-  HIP_TRY(launch_double_couette(h->d.storage, h->buf[h->cur], h->stride, h->nx, h->ny, h->nz, dim, dU, U, 0));
+  HIP_TRY(launch_double_couette(h->d.storage, h->buf[h->cur], h->stride, h->nx, h->ny, h->nz, dim, dU, U,
+                                dom_shift(h), 0));
   HIP_TRY(hipDeviceSynchronize());
   return HLBM_OK;
 }
@@ -773,7 +786,7 @@ int hlbm_set_populations_f64(hlbm_domain* h, const double* f, int32_t layout) {
       HIP_TRY(hipMemcpy(dst, src, (size_t)h->N * 8, hipMemcpyHostToDevice));
     } else {
       HIP_TRY(hipMemcpy(h->tmp, src, (size_t)h->N * 8, hipMemcpyHostToDevice));
-      HIP_TRY(launch_convert(h->d.storage, true, dst, h->tmp, h->N, i, 0));
+      HIP_TRY(launch_convert(h->d.storage, true, dst, h->tmp, h->N, i, dom_shift(h), 0));
     }
   }
   HIP_TRY(hipDeviceSynchronize());
@@ -841,7 +854,7 @@ int hlbm_get_populations_f64(hlbm_domain* h, double* f, int32_t layout) {
     if (h->d.storage == HLBM_F64) {
       HIP_TRY(hipMemcpy(dst, srcp, (size_t)h->N * 8, hipMemcpyDeviceToHost));
     } else {
-      HIP_TRY(launch_convert(h->d.storage, false, srcp, h->tmp, h->N, i, 0));
+      HIP_TRY(launch_convert(h->d.storage, false, srcp, h->tmp, h->N, i, dom_shift(h), 0));
       HIP_TRY(hipMemcpy(dst, h->tmp, (size_t)h->N * 8, hipMemcpyDeviceToHost));
     }
     if (layout == HLBM_LAYOUT_REF) {
@@ -994,9 +1007,27 @@ int hlbm_line_velocity(hlbm_domain* h, const int32_t start[3], int32_t axis, int
 int hlbm_set_collision(hlbm_domain* h, int32_t collision, const double* mrt_rates) {
   if (!h) return fail(HLBM_ERR_ARG, "NULL domain");
   if (collision != HLBM_BGK && collision != HLBM_MRT) return fail(HLBM_ERR_ARG, "collision must be BGK or MRT");
+  // the pending streaming belongs to the old collision; shifted storage then
+  // moves to the new rest state
+  {
+    const int rc = materialize_for_change(h);
+    if (rc) return rc;
+  }
+  const RestShift old = dom_shift(h);
   h->d.collision = collision;
   h->d.mrt_custom = mrt_rates ? 1 : 0;
   for (int k = 0; k < 10; k++) h->d.mrt_rates[k] = mrt_rates ? mrt_rates[k] : 0.0;
+  const RestShift now = dom_shift(h);
+  RestShift delta;
+  bool same = true;
+  for (int k = 0; k < 3; k++) {
+    delta.s[k] = old.s[k] - now.s[k];
+    same = same && delta.s[k] == 0.0;
+  }
+  if (!same && h->d.storage != HLBM_F64) {
+    HIP_TRY(launch_reshift(h->d.storage, h->buf[h->cur], h->stride, h->N, delta, 0));
+    HIP_TRY(hipDeviceSynchronize());
+  }
   return HLBM_OK;
 }
 

@@ -387,6 +387,10 @@ int hlbm_batch_submit(hlbm_batch* b, const hlbm_solve_desc* s, int64_t* id) {
     if (d.ef[i] != 0.0) d.has_force = 1;
   }
   d.omega_f = (float)(1.0 / d.tau);
+  {
+    const RestShift sh = rest_shift(s->collision == HLBM_MRT, s->mrt_custom, s->mrt_rates);
+    for (int k = 0; k < 3; k++) d.sig[k] = (float)sh.s[k];
+  }
   for (int k = 0; k < 10; k++) d.mrt[k] = s->mrt_rates[k];
   const double L = d.nz;  // cavity.hpp:58-63
   for (int side = 0; side < 2; side++) {
@@ -403,6 +407,11 @@ int hlbm_batch_submit(hlbm_batch* b, const hlbm_solve_desc* s, int64_t* id) {
   b->solves.emplace(v.id, std::move(v));
   b->queue.push_back(*id);
   return HLBM_OK;
+}
+
+// the rest state of a solve's shifted storage (RestShift)
+static RestShift solve_shift(const Solve& s) {
+  return rest_shift(s.desc.collision == HLBM_MRT, s.desc.mrt_custom, s.desc.mrt_rates);
 }
 
 static int admit_one(hlbm_batch* b, Solve& s) {
@@ -451,7 +460,8 @@ static int admit_one(hlbm_batch* b, Solve& s) {
   void* fa = (char*)b->arena[0] + s.f_off * b->esize;
   if (s.init_f.empty()) {
     int lo[3] = {0, 0, 0}, hi[3] = {s.dom.nx - 1, s.dom.ny - 1, s.dom.nz - 1};
-    if (launch_set_distribution(b->d.storage, fa, s.stride, s.dom.nx, s.dom.ny, lo, hi, s.desc.init_rho, st) !=
+    if (launch_set_distribution(b->d.storage, fa, s.stride, s.dom.nx, s.dom.ny, lo, hi, s.desc.init_rho,
+                                solve_shift(s), st) !=
         hipSuccess)
       return bfail(HLBM_ERR_HIP, "set_distribution");
   } else {
@@ -462,7 +472,7 @@ static int admit_one(hlbm_batch* b, Solve& s) {
       void* plane = (char*)fa + (size_t)i * s.stride * b->esize;
       if (hipMemcpyAsync(b->f_tmp, s.init_f.data() + (size_t)i * s.N, (size_t)s.N * 8, hipMemcpyHostToDevice, st) !=
               hipSuccess ||
-          launch_convert(b->d.storage, true, plane, b->f_tmp, s.N, i, st) != hipSuccess)
+          launch_convert(b->d.storage, true, plane, b->f_tmp, s.N, i, solve_shift(s), st) != hipSuccess)
         return bfail(HLBM_ERR_HIP, "initial populations");
     }
   }
@@ -507,7 +517,7 @@ static int retire_one(hlbm_batch* b, Solve& s) {
     s.f.resize((size_t)19 * s.N);
     for (int i = 0; i < Q; i++) {
       const void* plane = (const char*)fcur + (size_t)i * s.stride * b->esize;
-      if (launch_convert(b->d.storage, false, (void*)plane, b->f_tmp, s.N, i, st) != hipSuccess ||
+      if (launch_convert(b->d.storage, false, (void*)plane, b->f_tmp, s.N, i, solve_shift(s), st) != hipSuccess ||
           hipMemcpyAsync(s.f.data() + (size_t)i * s.N, b->f_tmp, s.N * 8, hipMemcpyDeviceToHost, st) != hipSuccess)
         return bfail(HLBM_ERR_HIP, "populations out");
     }

@@ -49,6 +49,30 @@ HLBM_HD constexpr int cavity_slot(int i) {
   return i == 5 || i == 6 ? 0 : i == 14 || i == 13 ? 1 : i == 11 || i == 12 ? 2 : i == 18 || i == 17 ? 3 : 4;
 }
 
+// Speed class of direction i: 0 rest, 1 axis, 2 diagonal.
+HLBM_HD constexpr int dir_class(int i) { return i == 0 ? 0 : (i < 7 ? 1 : 2); }
+
+// The rest state that shifted storage is relative to, g_i = f_i - w_i - sig_i,
+// per speed class.  BGK: sig = 0, the rest state is w.  MRT: hippoLBM's
+// relaxation drives the epsilon moment of a fluid at rest to weps (mrt.hpp:96-
+// 100; weps = 0 with the hard-coded rates) while w has epsilon = 3, so its
+// rest state is w + (weps - 3) * (column epsilon of the inverse moment matrix):
+// sig = (weps - 3) * {1/21, -1/63, 1/252}.  That column has no mass, momentum
+// or e-moment, so rho = 1 + sum g and j = sum c g hold for either shift.
+struct RestShift {
+  double s[3];
+};
+inline RestShift rest_shift(int mrt, int mrt_custom, const double* rates) {
+  RestShift r{{0.0, 0.0, 0.0}};
+  if (mrt) {
+    const double d = (mrt_custom ? rates[7] : 0.0) - 3.0;
+    r.s[0] = d / 21.0;
+    r.s[1] = -d / 63.0;
+    r.s[2] = d / 252.0;
+  }
+  return r;
+}
+
 // plane index (hlbm_plane) -> (dim, side)
 HLBM_HD constexpr int plane_dim(int p) { return p < 2 ? 2 : (p < 4 ? 1 : 0); }
 HLBM_HD constexpr int plane_side(int p) { return p & 1; }

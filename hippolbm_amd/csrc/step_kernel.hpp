@@ -98,6 +98,7 @@ struct StepParams {
   int32_t force_wide;   // take the 64-bit-address bulk variant even when 32-bit offsets would do
   int32_t pair32;       // (stride + N) * element size < 2^32: bounce-back waves gather without branches
   float omega_f;        // 1/tau for the FP32-compute forms
+  float sig[3];         // shifted storage: rest state w_i + sig[class(i)] (RestShift)
   float ef3f[Q];        // 3 c_i . F for the FP32-compute forms
   double cav[2][5];
   double tau;
@@ -523,8 +524,10 @@ __device__ __forceinline__ void mrt_collide(C* f, C rho_force, const StepParams&
     m[k] = moment<k>(f);
   });
   const MrtRates r = mrt_rates(p);
+  // shifted storage holds g = f - (w + sig): the rest state's moments are
+  // rho = 1, e = -11 and eps = weps (RestShift)
   if (shifted)
-    mrt_relax<C>(m, r, C(1), C(-11), C(3));
+    mrt_relax<C>(m, r, C(1), C(-11), C(r.weps));
   else
     mrt_relax<C>(m, r, C(0), C(0), C(0));
   static_for([&](auto ic) {
@@ -545,7 +548,7 @@ __device__ __forceinline__ void mrt_collide(C* f, C rho_force, const StepParams&
 // ---------------------------------------------------------------------------
 // lid_driven_cavity (lid_driven_cavity.hpp:52-62): f_i = feq(rho_local, U)
 template <class C, bool SHIFT>
-__device__ __forceinline__ void lid_node(C* f, const double* U) {
+__device__ __forceinline__ void lid_node(C* f, const double* U, const float* sig) {
   C rho = C(0);
 #pragma unroll
   for (int i = 0; i < Q; i++) rho += f[i];
@@ -562,7 +565,7 @@ __device__ __forceinline__ void lid_node(C* f, const double* U) {
     static_for([&](auto ic) {
       constexpr int i = decltype(ic)::value;
       const C eu = C(CX(i)) * Ux + C(CY(i)) * Uy + C(CZ(i)) * Uz;
-      f[i] = C(WD(i)) * (rho + rhot * (C(3.) * eu + C(4.5) * eu * eu - C(1.5) * u_squ));
+      f[i] = C(WD(i)) * (rho + rhot * (C(3.) * eu + C(4.5) * eu * eu - C(1.5) * u_squ)) - C(sig[dir_class(i)]);
     });
   }
 }
@@ -773,7 +776,7 @@ __device__ __forceinline__ void apply_program(const StepParams& p, int x, int y,
             });
           } else if (((bmask >> e.plane) & 1) && !wall) {
             if (e.kind == PK_LID) {
-              lid_node<C, SHIFT>(f, e.U);
+              lid_node<C, SHIFT>(f, e.U, p.sig);
             } else if (e.kind == PK_NEUMANN) {
               neumann_node<C, SHIFT>(f, e.plane, e.U);
             } else if (e.kind == PK_PRESSURE) {
@@ -939,7 +942,7 @@ __device__ __forceinline__ void apply_sets(const StepParams& p, int x, int y, in
       const double v = p.set_val[b];
 #pragma unroll
       for (int i = 0; i < Q; i++) {
-        const C nv = StoreOps<S>::shifted ? C(v * WD(i) - WD(i)) : C(v * WD(i));
+        const C nv = StoreOps<S>::shifted ? C(v * WD(i) - WD(i)) - C(p.sig[dir_class(i)]) : C(v * WD(i));
         f[i] = in ? nv : f[i];
       }
     }
@@ -1435,6 +1438,7 @@ struct BatchDomain {
   int32_t any_wall;      // the solve has obstacles (else no flag / mask is read)
   float omega_f;
   float ef3f[Q];
+  float sig[3];          // shifted storage: the solve's rest state (RestShift)
   double tau;
   double ef[Q];
   double cav[2][5];
@@ -1499,6 +1503,7 @@ __device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, c
   p.tau = d.tau;
   p.has_force = d.has_force;
   p.omega_f = d.omega_f;
+  for (int k = 0; k < 3; k++) p.sig[k] = d.sig[k];
   for (int i = 0; i < Q; i++) p.ef3f[i] = d.ef3f[i];
   for (int i = 0; i < Q; i++) p.ef[i] = d.ef[i];
   for (int k = 0; k < 10; k++) p.mrt[k] = d.mrt[k];

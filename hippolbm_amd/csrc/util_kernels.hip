@@ -173,7 +173,7 @@ __global__ void k_find_special(const uint8_t* flags, const uint8_t* gflags, int 
 // set_distribution (set_distribution.hpp:38-41): f_i = value * w_i on a node box.
 template <class S>
 __global__ void k_set_dist(void* fv, uint64_t stride, int nx, int ny, int lo0, int lo1, int lo2, int hi0, int hi1,
-                           int hi2, double value) {
+                           int hi2, double value, RestShift sh) {
   const int x = lo0 + blockIdx.x * blockDim.x + threadIdx.x;
   const int y = lo1 + blockIdx.y * blockDim.y + threadIdx.y;
   const int z = lo2 + blockIdx.z;
@@ -184,7 +184,7 @@ __global__ void k_set_dist(void* fv, uint64_t stride, int nx, int ny, int lo0, i
   for (int i = 0; i < Q; i++) {
     const double v = value * WD(i);
     if constexpr (StoreOps<S>::shifted)
-      StoreOps<S>::st(f, (size_t)i * stride + idx, (float)(v - WD(i)));
+      StoreOps<S>::st(f, (size_t)i * stride + idx, (float)(v - WD(i) - sh.s[dir_class(i)]));
     else
       StoreOps<S>::st(f, (size_t)i * stride + idx, v);
   }
@@ -195,7 +195,7 @@ __global__ void k_set_dist(void* fv, uint64_t stride, int nx, int ny, int lo0, i
 // in the reference's evaluation order.
 template <class S>
 __global__ void k_double_couette(void* fv, uint64_t stride, int nx, int ny, int nz, int dim, double dUx, double dUy,
-                                 double dUz, double Ux, double Uy, double Uz) {
+                                 double dUz, double Ux, double Uy, double Uz, RestShift sh) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
   const int z = blockIdx.z;
@@ -210,7 +210,7 @@ __global__ void k_double_couette(void* fv, uint64_t stride, int nx, int ny, int 
     const double eu = ux * CX(i) + uy * CY(i) + uz * CZ(i);
     const double v = 1. * WD(i) * (1. + 3. * eu + 4.5 * eu * eu - 1.5 * u_squ);
     if constexpr (StoreOps<S>::shifted)
-      StoreOps<S>::st(f, (size_t)i * stride + idx, (float)(v - WD(i)));
+      StoreOps<S>::st(f, (size_t)i * stride + idx, (float)(v - WD(i) - sh.s[dir_class(i)]));
     else
       StoreOps<S>::st(f, (size_t)i * stride + idx, v);
   }
@@ -218,47 +218,47 @@ __global__ void k_double_couette(void* fv, uint64_t stride, int nx, int ny, int 
 
 template <class S>
 static hipError_t double_couette_t(void* f, uint64_t stride, int nx, int ny, int nz, int dim, const double* dU,
-                                   const double* U, hipStream_t s) {
+                                   const double* U, RestShift sh, hipStream_t s) {
   dim3 b(64, 4, 1), g((nx + 63) / 64, (ny + 3) / 4, nz);
   hipLaunchKernelGGL((k_double_couette<S>), g, b, 0, s, f, stride, nx, ny, nz, dim, dU[0], dU[1], dU[2], U[0], U[1],
-                     U[2]);
+                     U[2], sh);
   return hipGetLastError();
 }
 
 hipError_t launch_double_couette(int storage, void* f, uint64_t stride, int nx, int ny, int nz, int dim,
-                                 const double* dU, const double* U, hipStream_t s) {
+                                 const double* dU, const double* U, RestShift sh, hipStream_t s) {
   switch (storage) {
     case 0:
-      return double_couette_t<double>(f, stride, nx, ny, nz, dim, dU, U, s);
+      return double_couette_t<double>(f, stride, nx, ny, nz, dim, dU, U, sh, s);
     case 1:
-      return double_couette_t<float>(f, stride, nx, ny, nz, dim, dU, U, s);
+      return double_couette_t<float>(f, stride, nx, ny, nz, dim, dU, U, sh, s);
     case 2:
-      return double_couette_t<_Float16>(f, stride, nx, ny, nz, dim, dU, U, s);
+      return double_couette_t<_Float16>(f, stride, nx, ny, nz, dim, dU, U, sh, s);
     case 3:
-      return double_couette_t<bf16s>(f, stride, nx, ny, nz, dim, dU, U, s);
+      return double_couette_t<bf16s>(f, stride, nx, ny, nz, dim, dU, U, sh, s);
   }
   return hipErrorInvalidValue;
 }
 
 // FP64 plane -> storage plane (adds the shift)
 template <class S>
-__global__ void k_from_f64(void* fv, const double* src, uint32_t n, int q) {
+__global__ void k_from_f64(void* fv, const double* src, uint32_t n, int q, double sig) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   S* f = (S*)fv;
   if constexpr (StoreOps<S>::shifted)
-    StoreOps<S>::st(f, i, (float)(src[i] - WD(q)));
+    StoreOps<S>::st(f, i, (float)(src[i] - WD(q) - sig));
   else
     StoreOps<S>::st(f, i, src[i]);
 }
 
 template <class S>
-__global__ void k_to_f64(double* dst, const void* fv, uint32_t n, int q) {
+__global__ void k_to_f64(double* dst, const void* fv, uint32_t n, int q, double sig) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const S* f = (const S*)fv;
   if constexpr (StoreOps<S>::shifted)
-    dst[i] = (double)StoreOps<S>::ld(f, i) + WD(q);
+    dst[i] = (double)StoreOps<S>::ld(f, i) + (WD(q) + sig);
   else
     dst[i] = StoreOps<S>::ld(f, i);
 }
@@ -571,51 +571,84 @@ hipError_t launch_find_special(const uint8_t* flags, const uint8_t* gflags, int 
 
 template <class S>
 static hipError_t set_dist_t(void* f, uint64_t stride, int nx, int ny, const int* lo, const int* hi, double value,
-                             hipStream_t s) {
+                             RestShift sh, hipStream_t s) {
   const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
   dim3 b(64, 4, 1), g((ex + 63) / 64, (ey + 3) / 4, ez);
   hipLaunchKernelGGL((k_set_dist<S>), g, b, 0, s, f, stride, nx, ny, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2],
-                     value);
+                     value, sh);
   return hipGetLastError();
 }
 
 hipError_t launch_set_distribution(int storage, void* f, uint64_t stride, int nx, int ny, const int* lo,
-                                   const int* hi, double value, hipStream_t s) {
+                                   const int* hi, double value, RestShift sh, hipStream_t s) {
   switch (storage) {
     case 0:
-      return set_dist_t<double>(f, stride, nx, ny, lo, hi, value, s);
+      return set_dist_t<double>(f, stride, nx, ny, lo, hi, value, sh, s);
     case 1:
-      return set_dist_t<float>(f, stride, nx, ny, lo, hi, value, s);
+      return set_dist_t<float>(f, stride, nx, ny, lo, hi, value, sh, s);
     case 2:
-      return set_dist_t<_Float16>(f, stride, nx, ny, lo, hi, value, s);
+      return set_dist_t<_Float16>(f, stride, nx, ny, lo, hi, value, sh, s);
     case 3:
-      return set_dist_t<bf16s>(f, stride, nx, ny, lo, hi, value, s);
+      return set_dist_t<bf16s>(f, stride, nx, ny, lo, hi, value, sh, s);
   }
   return hipErrorInvalidValue;
 }
 
 template <class S>
-static hipError_t conv_t(bool to_store, void* f, double* tmp, uint32_t n, int q, hipStream_t s) {
+static hipError_t conv_t(bool to_store, void* f, double* tmp, uint32_t n, int q, double sig, hipStream_t s) {
   dim3 b(256), g((n + 255) / 256);
   if (to_store)
-    hipLaunchKernelGGL((k_from_f64<S>), g, b, 0, s, f, (const double*)tmp, n, q);
+    hipLaunchKernelGGL((k_from_f64<S>), g, b, 0, s, f, (const double*)tmp, n, q, sig);
   else
-    hipLaunchKernelGGL((k_to_f64<S>), g, b, 0, s, tmp, (const void*)f, n, q);
+    hipLaunchKernelGGL((k_to_f64<S>), g, b, 0, s, tmp, (const void*)f, n, q, sig);
   return hipGetLastError();
 }
 
-hipError_t launch_convert(int storage, bool to_store, void* fplane, double* tmp, uint32_t n, int q, hipStream_t s) {
+hipError_t launch_convert(int storage, bool to_store, void* fplane, double* tmp, uint32_t n, int q, RestShift sh,
+                          hipStream_t s) {
+  const double sig = sh.s[dir_class(q)];
   switch (storage) {
     case 0:
-      return conv_t<double>(to_store, fplane, tmp, n, q, s);
+      return conv_t<double>(to_store, fplane, tmp, n, q, sig, s);
     case 1:
-      return conv_t<float>(to_store, fplane, tmp, n, q, s);
+      return conv_t<float>(to_store, fplane, tmp, n, q, sig, s);
     case 2:
-      return conv_t<_Float16>(to_store, fplane, tmp, n, q, s);
+      return conv_t<_Float16>(to_store, fplane, tmp, n, q, sig, s);
     case 3:
-      return conv_t<bf16s>(to_store, fplane, tmp, n, q, s);
+      return conv_t<bf16s>(to_store, fplane, tmp, n, q, sig, s);
   }
   return hipErrorInvalidValue;
+}
+
+// Shifted storage moving to another rest state: g += delta[class(i)].
+template <class S>
+__global__ void k_reshift(void* fv, uint64_t stride, uint32_t N, RestShift delta) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= N) return;
+  S* f = (S*)fv;
+#pragma unroll
+  for (int i = 0; i < Q; i++) {
+    const size_t k = (size_t)i * stride + idx;
+    StoreOps<S>::st(f, k, StoreOps<S>::ld(f, k) + (float)delta.s[dir_class(i)]);
+  }
+}
+
+hipError_t launch_reshift(int storage, void* f, uint64_t stride, uint32_t N, RestShift delta, hipStream_t s) {
+  dim3 b(256), g((N + 255) / 256);
+  switch (storage) {
+    case 1:
+      hipLaunchKernelGGL((k_reshift<float>), g, b, 0, s, f, stride, N, delta);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_reshift<_Float16>), g, b, 0, s, f, stride, N, delta);
+      break;
+    case 3:
+      hipLaunchKernelGGL((k_reshift<bf16s>), g, b, 0, s, f, stride, N, delta);
+      break;
+    default:
+      return hipSuccess;  // FP64 storage is not shifted
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_macro(int storage, const void* f, uint64_t stride, uint32_t N, const uint8_t* flags, double* m0,

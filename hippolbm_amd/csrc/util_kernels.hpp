@@ -87,11 +87,16 @@ hipError_t launch_ghost_images(uint8_t* gflags, const uint8_t* flags, int nx, in
 hipError_t launch_find_special(const uint8_t* flags, const uint8_t* gflags, int nx, int ny, int nz, const int* per,
                                uint32_t* bbm, uint32_t* out_idx, uint32_t* out_mask, uint32_t* count, uint32_t cap,
                                hipStream_t s);
+// The initial-state and conversion kernels write and read shifted storage
+// relative to the rest state sh (RestShift, d3q19.hpp); FP64 storage ignores it.
 hipError_t launch_set_distribution(int storage, void* f, uint64_t stride, int nx, int ny, const int* lo,
-                                   const int* hi, double value, hipStream_t s);
+                                   const int* hi, double value, RestShift sh, hipStream_t s);
 hipError_t launch_double_couette(int storage, void* f, uint64_t stride, int nx, int ny, int nz, int dim,
-                                 const double* dU, const double* U, hipStream_t s);
-hipError_t launch_convert(int storage, bool to_store, void* fplane, double* tmp, uint32_t n, int q, hipStream_t s);
+                                 const double* dU, const double* U, RestShift sh, hipStream_t s);
+hipError_t launch_convert(int storage, bool to_store, void* fplane, double* tmp, uint32_t n, int q, RestShift sh,
+                          hipStream_t s);
+// shifted storage: g_i += delta[class(i)] over N nodes (a change of rest state)
+hipError_t launch_reshift(int storage, void* f, uint64_t stride, uint32_t N, RestShift delta, hipStream_t s);
 hipError_t launch_macro(int storage, const void* f, uint64_t stride, uint32_t N, const uint8_t* flags, double* m0,
                         double* m1, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s);

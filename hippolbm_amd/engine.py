@@ -102,6 +102,12 @@ class Domain:
         buf = (BC * max(1, len(arr)))(*arr)
         check("hlbm_set_bcs", lib().hlbm_set_bcs(self.h, buf, len(arr)))
 
+    def set_collision(self, collision, mrt_rates=None):
+        """The collision slot: "bgk" or "mrt" (mrt_rates None = hippoLBM's rates)."""
+        r = None if mrt_rates is None else np.ascontiguousarray(mrt_rates, dtype=np.float64)
+        check("hlbm_set_collision", lib().hlbm_set_collision(self.h, _lib.COLLISION[collision],
+                                                             _dp(r) if r is not None else None))
+
     def set_params(self, tau, fext):
         F = np.ascontiguousarray(fext, dtype=np.float64)
         check("hlbm_set_params", lib().hlbm_set_params(self.h, float(tau), _dp(F)))

@@ -100,6 +100,9 @@ typedef struct {
   double fext[3];      /* LBMParameters::Fext_ (LBM units)                                     */
   double dx;           /* grid spacing, used by obstacle rasterization                         */
   double mrt_rates[10];/* s1, s2, s4, s9, s10, s13, s16, weps, wepsj, wxx                     */
+  /* FP32/FP16/BF16 storage holds f_i minus the collision's rest state: w_i for
+   * BGK, w_i + (weps - 3) * {1/21, -1/63, 1/252} (rest/axis/diagonal) for MRT,
+   * whose rates relax the epsilon moment at rest to weps (mrt.hpp:96-100). */
   int32_t round_nearest;/* FP16/BF16 storage: 0 = stochastic rounding of every stored
                            population (the default: unbiased, so rounding errors do not
                            accumulate in steady flow), 1 = round to nearest even          */
@@ -152,7 +155,8 @@ int hlbm_set_params(hlbm_domain* h, double tau, const double fext[3]);
 /* Select the collision operator the `collision` slot names: HLBM_BGK
  * (src/collision/bgk.cu:77-94) or HLBM_MRT (src/collision/mrt.cu:71-89);
  * mrt_rates = NULL keeps the reference's hard-coded rates (mrt.hpp:44-47),
- * otherwise [s1 s2 s4 s9 s10 s13 s16 weps wepsj wxx]. */
+ * otherwise [s1 s2 s4 s9 s10 s13 s16 weps wepsj wxx].  A pending streaming is
+ * applied first; shifted storage moves to the new rest state. */
 int hlbm_set_collision(hlbm_domain* h, int32_t collision, const double* mrt_rates);
 
 /* ---- initial state ----------------------------------------------------- */

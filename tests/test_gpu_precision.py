@@ -10,12 +10,14 @@ from hippolbm_amd.engine import Simulation
 pytestmark = pytest.mark.gpu
 
 # relative L2 error of u and max |rho error| after 400 steps of a 32^3 moving-lid
-# Couette, per (storage, collision).  FP16 BGK meets the 1e-3 velocity target;
-# MRT with hippoLBM's rates (weps = 0) drives the rest state away from w, so its
-# shifted populations are not small and FP16/BF16 storage loses more.
-TOL = {("f32", "bgk"): (1e-5, 1e-6), ("f32", "mrt"): (1e-5, 1e-5),
-       ("f16", "bgk"): (1e-3, 1e-4), ("f16", "mrt"): (5e-2, 1e-3),
-       ("bf16", "bgk"): (5e-2, 1e-3), ("bf16", "mrt"): (5e-1, 1e-2)}
+# Couette, per (storage, collision): about 3x the values measured on MI355X
+# (FP32 9.9e-7 / 2.7e-6, FP16 1.5e-4 / 1.6e-4, BF16 1.2e-3 / 1.3e-3 for BGK /
+# MRT; stochastic rounding is a deterministic hash, so the figures repeat).
+# MRT storage is shifted by MRT's own rest state, so FP16/BF16 MRT lose no more
+# than BGK.
+TOL = {("f32", "bgk"): (3e-6, 1e-13), ("f32", "mrt"): (8e-6, 4e-7),
+       ("f16", "bgk"): (5e-4, 8e-5), ("f16", "mrt"): (5e-4, 7e-5),
+       ("bf16", "bgk"): (4e-3, 6e-4), ("bf16", "mrt"): (4e-3, 1.3e-3)}
 
 
 def deck(n, collision="bgk"):
@@ -133,3 +135,37 @@ def test_stochastic_rounding_is_deterministic():
     assert np.array_equal(a.dom.get_f(), b.dom.get_f())
     a.dom.close()
     b.dom.close()
+
+
+def test_f16_mrt_long_run_meets_the_velocity_target():
+    # MRT storage is shifted by MRT's own rest state (w plus the epsilon-moment
+    # correction that weps = 0 implies), so the stored deviations stay small
+    d = deck(64, "mrt")
+    ref = Simulation(d, storage="f64")
+    low = Simulation(d, storage="f16")
+    ref.run(2000)
+    low.run(2000)
+    r0, u0 = ref.dom.macro()
+    r1, u1 = low.dom.macro()
+    rel_l2 = np.linalg.norm(u1 - u0) / np.linalg.norm(u0)
+    print(f"f16 mrt 64^3 2000 steps: rel L2(u) = {rel_l2:.3e}, max|drho| = {np.abs(r1 - r0).max():.3e}", flush=True)
+    assert rel_l2 <= 1e-3
+    ref.dom.close()
+    low.dom.close()
+
+
+@pytest.mark.parametrize("storage", ["f32", "f16"])
+def test_changing_the_collision_moves_the_rest_state(storage):
+    # a shifted-storage domain switched between BGK and MRT keeps its populations
+    # (its storage moves to the new rest state)
+    d = deck(16, "bgk")
+    s = Simulation(d, storage=storage)
+    s.run(10)
+    f0 = s.dom.get_f()
+    s.dom.set_collision("mrt")
+    f1 = s.dom.get_f()
+    s.dom.set_collision("bgk")
+    f2 = s.dom.get_f()
+    tol = 1e-7 if storage == "f32" else 2e-4
+    assert np.abs(f1 - f0).max() < tol and np.abs(f2 - f0).max() < tol
+    s.dom.close()
