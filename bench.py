@@ -76,7 +76,7 @@ def measure(n, storage, steps, warmup, collision="bgk", tune=True, pattern="ab")
             "launch": tuned}
 
 
-def porous_channel(steps=100, warmup=10, storage="f64"):
+def porous_channel(steps=100, warmup=10, storage="f64", tune=True):
     """The porous channel 256x128x128 (MRT, s_nu = 1/0.7, other rates 1.2;
     density 1.001 inlet, zero-gradient outflow, bounce-back y/z walls, 150 seeded
     non-overlapping spheres r ~ U[5,12]) on the A/B kernel."""
@@ -84,7 +84,7 @@ def porous_channel(steps=100, warmup=10, storage="f64"):
     from hippolbm_amd.engine import Simulation
     d = workloads.porous_channel()
     sim = Simulation(d, storage=storage)
-    tuned = sim.dom.autotune(reps=3)
+    tuned = sim.dom.autotune(reps=3) if tune else None
     ms, _ = time_steps(sim.dom, steps, warmup)
     flags = sim.dom.get_flags()
     nodes = sim.dom.N
@@ -382,7 +382,7 @@ def main():
         print(json.dumps({"output_operators": output_operators(args.size)}), flush=True)
         return 0
     if args.porous:
-        print(json.dumps({"porous_channel": porous_channel(args.steps, args.warmup)}), flush=True)
+        print(json.dumps({"porous_channel": porous_channel(args.steps, args.warmup, tune=not args.no_tune)}), flush=True)
         return 0
     if args.precision:
         print(json.dumps({"precision_sweep": precision_sweep(args.precision)}), flush=True)

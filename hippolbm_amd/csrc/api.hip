@@ -111,6 +111,7 @@ static StepParams base_params(hlbm_domain* h) {
   p.m0 = h->m0;
   p.m1 = h->m1;
   p.tau = h->d.tau;
+  p.inv_tau = 1 / h->d.tau;
   for (int d = 0; d < 3; d++) p.F[d] = h->d.fext[d];
   // c_i . F in the reference's order: ex*Fx + ey*Fy + ez*Fz (bgk.hpp:57, mrt.hpp:188)
   for (int i = 0; i < Q; i++) p.ef[i] = CX(i) * h->d.fext[0] + CY(i) * h->d.fext[1] + CZ(i) * h->d.fext[2];

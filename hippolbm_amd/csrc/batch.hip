@@ -178,6 +178,8 @@ static StepParams single_params(hlbm_batch* b, const Solve& s, int parity) {
     p.cav[1][k] = s.dom.cav[1][k];
   }
   p.tau = s.dom.tau;
+  p.inv_tau = s.dom.inv_tau;
+  for (int k = 0; k < 3; k++) p.sig[k] = s.dom.sig[k];
   for (int i = 0; i < Q; i++) p.ef[i] = s.dom.ef[i];
   set_fp32_constants(p);
   for (int k = 0; k < 10; k++) p.mrt[k] = s.dom.mrt[k];
@@ -380,6 +382,7 @@ int hlbm_batch_submit(hlbm_batch* b, const hlbm_solve_desc* s, int64_t* id) {
   d.inv_nx = 1.0 / (double)d.nx;
   d.inv_nxy = 1.0 / ((double)d.nx * (double)d.ny);
   d.tau = s->tau;
+  d.inv_tau = 1 / s->tau;
   for (int i = 0; i < Q; i++) d.ef[i] = CX(i) * s->fext[0] + CY(i) * s->fext[1] + CZ(i) * s->fext[2];
   d.has_force = 0;
   for (int i = 0; i < Q; i++) {

@@ -102,6 +102,7 @@ struct StepParams {
   float ef3f[Q];        // 3 c_i . F for the FP32-compute forms
   double cav[2][5];
   double tau;
+  double inv_tau;       // 1 / tau (the MRT default s9 = s13, mrt.hpp:43)
   double F[3];
   double ef[Q];         // c_i . F, evaluated in the reference's order on the host
   double mrt[10];       // s1, s2, s4, s9, s10, s13, s16, weps, wepsj, wxx (mrt_custom)
@@ -293,9 +294,21 @@ __device__ __forceinline__ void static_for(F&& fn) {
 // ---------------------------------------------------------------------------
 // BGK (bgk.hpp:47-62), FP64 order-faithful
 // ---------------------------------------------------------------------------
+// a / tau, correctly rounded, from y = RN(1/tau) (an IEEE division on the host):
+// q = RN(a y) is within one ulp of a/tau, r = a - q tau is exact in one fused
+// multiply-add, and RN(q + r y) is the correctly rounded quotient (Markstein,
+// IBM J. Res. Dev. 34 (1990) 111-119; a CPU check of 4.4e8 quotients against
+// IEEE division found no difference).  Three instructions instead of the
+// general division's scale / reciprocal / refinement sequence.  (For a = -0 it
+// returns +0 where IEEE gives -0; f + (+-0) is the same for every f != -0.)
+__device__ __forceinline__ double div_tau(double a, const StepParams& p) {
+  const double q = a * p.inv_tau;
+  const double r = __builtin_fma(-q, p.tau, a);
+  return __builtin_fma(r, p.inv_tau, q);
+}
+
 __device__ __forceinline__ void bgk_f64(double* f, double rho, double ux, double uy, double uz, const StepParams& p) {
   const double u_squ = (ux * ux + uy * uy + uz * uz);
-  const double tau = p.tau;
   static_for([&](auto ic) {
     constexpr int i = decltype(ic)::value;
     const double w = WD(i);
@@ -304,9 +317,9 @@ __device__ __forceinline__ void bgk_f64(double* f, double rho, double ux, double
     // F = 0 makes the force term exactly +0 (c_i . F is evaluated to +0.0), so
     // dropping it leaves every sum bit-identical
     if (p.has_force)
-      f[i] += (feq - f[i]) / tau + 3. * rho * w * p.ef[i];
+      f[i] += div_tau(feq - f[i], p) + 3. * rho * w * p.ef[i];
     else
-      f[i] += (feq - f[i]) / tau;
+      f[i] += div_tau(feq - f[i], p);
   });
 }
 
@@ -399,6 +412,20 @@ struct MrtBasis {
   };
 };
 
+constexpr bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// acc +- c * v for a coefficient c = sgn * 2^e: the product is exact, so one
+// fused multiply-add rounds exactly as the separate product and sum do.  (For
+// populations and moments of a lattice fluid the product never reaches the
+// subnormal range, where it would stop being exact.)
+template <class C>
+__device__ __forceinline__ C acc_exact(C acc, C c, C v) {
+  if constexpr (std::is_same<C, double>::value)
+    return __builtin_fma(c, v, acc);
+  else
+    return acc + c * v;
+}
+
 // Row k of M applied to f, accumulated left to right over the non-zero entries.
 template <int k, class C>
 __device__ __forceinline__ C moment(const C* f) {
@@ -408,15 +435,16 @@ __device__ __forceinline__ C moment(const C* f) {
     constexpr int i = decltype(ic)::value;
     constexpr int c = MrtBasis::M[k][i];
     if constexpr (c != 0) {
-      C term;
-      if constexpr (c == 1 || c == -1)
-        term = f[i];
-      else
-        term = C(c < 0 ? -c : c) * f[i];
+      constexpr int a = c < 0 ? -c : c;
       if (first) {
-        acc = c < 0 ? -term : term;
+        acc = a == 1 ? (c < 0 ? -f[i] : f[i]) : C(c) * f[i];
         first = false;
+      } else if constexpr (a == 1) {
+        acc = c < 0 ? acc - f[i] : acc + f[i];
+      } else if constexpr (pow2(a)) {
+        acc = acc_exact(acc, C(c), f[i]);
       } else {
+        const C term = C(a) * f[i];
         acc = c < 0 ? acc - term : acc + term;
       }
     }
@@ -429,20 +457,27 @@ template <int i, class C>
 __device__ __forceinline__ C inverse_row(const C* mo) {
   C acc = C(0);
   bool first = true;
-#pragma unroll
-  for (int k = 0; k < 19; k++) {
-    const int num = MrtBasis::NUM[i][k];
-    if (num != 0) {
-      const C coef = C((double)(num < 0 ? -num : num) / (double)MrtBasis::DEN[i][k]);
-      const C term = coef * mo[k];
+  static_for([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    constexpr int num = MrtBasis::NUM[i][k];
+    if constexpr (num != 0) {
+      constexpr int an = num < 0 ? -num : num;
+      constexpr int den = MrtBasis::DEN[i][k];
+      // |num|/den is a power of two (1, 1/4, 1/8) when den = |num| * 2^e
+      constexpr bool exact = den % an == 0 && pow2(den / an);
+      const C coef = C((double)an / (double)den);
       if (first) {
+        const C term = coef * mo[k];
         acc = num < 0 ? -term : term;
         first = false;
+      } else if constexpr (exact) {
+        acc = acc_exact(acc, num < 0 ? -coef : coef, mo[k]);
       } else {
+        const C term = coef * mo[k];
         acc = num < 0 ? acc - term : acc + term;
       }
     }
-  }
+  });
   return acc;
 }
 
@@ -463,8 +498,8 @@ __device__ __forceinline__ MrtRates mrt_rates(const StepParams& p) {
     r.weps = p.mrt[7];
     r.wepsj = p.mrt[8];
     r.wxx = p.mrt[9];
-  } else {  // mrt.hpp:43-47
-    r.s9 = 1 / p.tau;
+  } else {  // mrt.hpp:43-47; 1 / tau computed once on the host (IEEE division either way)
+    r.s9 = p.inv_tau;
     r.s13 = r.s9;
     r.s1 = 1.19;
     r.s2 = 1.4;
@@ -1440,6 +1475,7 @@ struct BatchDomain {
   float ef3f[Q];
   float sig[3];          // shifted storage: the solve's rest state (RestShift)
   double tau;
+  double inv_tau;
   double ef[Q];
   double cav[2][5];
   double mrt[10];
@@ -1501,6 +1537,7 @@ __device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, c
     p.cav[1][k] = d.cav[1][k];
   }
   p.tau = d.tau;
+  p.inv_tau = d.inv_tau;
   p.has_force = d.has_force;
   p.omega_f = d.omega_f;
   for (int k = 0; k < 3; k++) p.sig[k] = d.sig[k];
