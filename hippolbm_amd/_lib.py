@@ -33,8 +33,8 @@ SYMBOLS = (
     "hlbm_set_flags", "hlbm_get_flags", "hlbm_set_distribution", "hlbm_set_populations_f64",
     "hlbm_get_populations_f64", "hlbm_step", "hlbm_timestep", "hlbm_synchronize", "hlbm_macro", "hlbm_get_macro",
     "hlbm_reduce_stats", "hlbm_ghost_update_ref", "hlbm_copy_bench", "hlbm_autotune", "hlbm_set_launch",
-    "hlbm_set_collision", "hlbm_plane_profile", "hlbm_plane_profile_noalloc", "hlbm_line_velocity",
-    "hlbm_init_double_couette",
+    "hlbm_set_collision", "hlbm_set_layout", "hlbm_get_layout", "hlbm_plane_profile", "hlbm_plane_profile_noalloc",
+    "hlbm_line_velocity", "hlbm_init_double_couette",
     "hlbm_batch_create", "hlbm_batch_destroy", "hlbm_batch_submit", "hlbm_batch_run", "hlbm_batch_status",
     "hlbm_batch_times", "hlbm_batch_result", "hlbm_batch_stats", "hlbm_batch_info_get", "hlbm_clock",
 )
@@ -110,7 +110,7 @@ class TuneInfo(ctypes.Structure):
         ("candidates", ctypes.c_int32),
         ("variant", ctypes.c_int32),
         ("relocated", ctypes.c_int32),
-        ("pad", ctypes.c_int32),
+        ("order", ctypes.c_int32),
         ("ms_best", ctypes.c_double),
         ("ms_default", ctypes.c_double),
     ]
@@ -165,6 +165,8 @@ def lib():
         "hlbm_set_launch": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
                                            ctypes.c_int32]),
         "hlbm_set_collision": (ctypes.c_int, [P, ctypes.c_int32, DP]),
+        "hlbm_set_layout": (ctypes.c_int, [P, I32P]),
+        "hlbm_get_layout": (ctypes.c_int, [P, I32P]),
         "hlbm_plane_profile": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_double, DP, I32P, DP, DP]),
         "hlbm_plane_profile_noalloc": (ctypes.c_int, [P, ctypes.c_int32, ctypes.c_double, P, P]),
         "hlbm_line_velocity": (ctypes.c_int, [P, I32P, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, DP]),

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <mutex>
 #include <cmath>
@@ -82,16 +83,47 @@ struct hlbm_domain {
   int boundary_first = 1;  // program-plane kernel before the bulk kernel (measured 1-2 % faster)
   int device = 0;
   uint64_t bytes = 0;
+  NodeLayout lay{};  // node layout in HBM (hlbm_set_layout; default x fastest)
 };
 
+// nodes along the k-th axis of the domain's layout (0 = fastest)
+static int along(const hlbm_domain* h, int k) {
+  const int n[3] = {h->nx, h->ny, h->nz};
+  return n[h->lay.ord[k]];
+}
+
+static bool default_layout_p(const hlbm_domain* h) {
+  return h->lay.ord[0] == 0 && h->lay.ord[1] == 1 && h->lay.ord[2] == 2;
+}
+
+// Host copies between the ABI's node order (x fastest) and the domain's layout.
+template <class T>
+static void to_layout(const hlbm_domain* h, const T* src, T* dst) {
+  size_t k = 0;
+  for (int z = 0; z < h->nz; z++)
+    for (int y = 0; y < h->ny; y++)
+      for (int x = 0; x < h->nx; x++, k++) dst[node_at(h->lay, x, y, z)] = src[k];
+}
+template <class T>
+static void from_layout(const hlbm_domain* h, const T* src, T* dst) {
+  size_t k = 0;
+  for (int z = 0; z < h->nz; z++)
+    for (int y = 0; y < h->ny; y++)
+      for (int x = 0; x < h->nx; x++, k++) dst[k] = src[node_at(h->lay, x, y, z)];
+}
+
+// bulk-kernel tiles: block.x nodes along the layout's fastest axis by
+// block.y * npt rows along the next one; the grid walks the slowest axis
+static int eff_npt(const hlbm_domain* h) { return default_layout_p(h) ? h->npt : 1; }
+
 static unsigned tiles_y(const hlbm_domain* h) {
-  const unsigned rows = h->block.y * h->npt;
-  return (h->ny + rows - 1) / rows;
+  const unsigned rows = h->block.y * eff_npt(h);
+  return (along(h, 1) + rows - 1) / rows;
 }
 
 static dim3 grid_for(const hlbm_domain* h) {
-  const unsigned gx = (h->nx + h->block.x - 1) / h->block.x;
-  return dim3(gx * tiles_y(h) * h->nz, 1, 1);
+  const unsigned gx = (along(h, 0) + h->block.x - 1) / h->block.x;
+  return dim3(gx * tiles_y(h) * along(h, 2), 1, 1);
 }
 
 // The rest state the domain's shifted storage is relative to (RestShift).
@@ -121,9 +153,11 @@ static StepParams base_params(hlbm_domain* h) {
     for (int k = 0; k < 3; k++) p.sig[k] = (float)sh.s[k];
   }
   p.mrt_custom = h->d.mrt_custom;
-  p.gx = (int)((h->nx + h->block.x - 1) / h->block.x);
+  p.lay = h->lay;
+  for (int k = 0; k < 3; k++) p.ln[k] = along(h, k);
+  p.gx = (int)((along(h, 0) + h->block.x - 1) / h->block.x);
   p.gy = (int)tiles_y(h);
-  p.gz = h->nz;
+  p.gz = along(h, 2);
   p.xcd_remap = h->xcd_remap;
   for (int k = 0; k < 10; k++) p.mrt[k] = h->d.mrt_rates[k];
   if (!h->any_wall) p.wall_bb = 0;
@@ -175,7 +209,7 @@ static int build_special(hlbm_domain* h, hipStream_t s) {
   uint32_t count = 0;
   HIP_TRY(hipMemsetAsync(h->spec_count, 0, 4, s));
   HIP_TRY(launch_find_special(h->flags, h->gflags, h->nx, h->ny, h->nz, per, h->bbm, h->spec_idx, h->spec_mask,
-                              h->spec_count, h->spec_cap, s));
+                              h->spec_count, h->spec_cap, h->lay, s));
   HIP_TRY(hipMemcpyAsync(&count, h->spec_count, 4, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
   if (count > h->spec_cap) return fail(HLBM_ERR_HIP, "periodic-face wall list overflow");
@@ -220,8 +254,13 @@ static int check_special(hlbm_domain* h) {
                   "position, is a wall while its periodic image is fluid (hippoLBM then streams the wall's own "
                   "populations, which this engine does not keep)");
     const uint32_t idx = h->spec_host_idx[k];
-    const int c[3] = {(int)(idx % (uint32_t)h->nx), (int)(idx / (uint32_t)h->nx % (uint32_t)h->ny),
-                      (int)(idx / ((uint32_t)h->nx * (uint32_t)h->ny))};
+    int c[3];
+    {
+      const uint32_t n0 = (uint32_t)along(h, 0), n01 = n0 * (uint32_t)along(h, 1);
+      c[h->lay.ord[2]] = (int)(idx / n01);
+      c[h->lay.ord[1]] = (int)(idx % n01 / n0);
+      c[h->lay.ord[0]] = (int)(idx % n0);
+    }
     int planes = 0;
     for (int pl = 0; pl < 6; pl++) {
       const int d = plane_dim(pl);
@@ -251,7 +290,7 @@ static int ensure_bbm(hlbm_domain* h, hipStream_t s) {
     h->bytes += (uint64_t)h->N * 4;
   }
   const int per[3] = {h->d.periodic[0] ? 1 : 0, h->d.periodic[1] ? 1 : 0, h->d.periodic[2] ? 1 : 0};
-  hipError_t e = launch_build_bbm(h->flags, h->nx, h->ny, h->nz, per, h->bbm, s);
+  hipError_t e = launch_build_bbm(h->flags, h->nx, h->ny, h->nz, per, h->bbm, h->lay, s);
   if (e != hipSuccess) return fail(HLBM_ERR_HIP, std::string("bounce-back mask: ") + hipGetErrorString(e));
   const int rc = build_special(h, s);
   if (rc) return rc;
@@ -359,6 +398,7 @@ int hlbm_domain_create(const hlbm_domain_desc* desc, hlbm_domain** out) {
   h->ny = desc->nodes[1];
   h->nz = desc->nodes[2];
   h->N = (uint32_t)N64;
+  h->lay = default_layout(h->nx, h->ny, h->nz);
   h->esize = elem_size(desc->storage);
   const uint64_t align = 256 / h->esize;
   h->stride = (N64 + align - 1) / align * align;
@@ -405,8 +445,8 @@ int hlbm_domain_create(const hlbm_domain_desc* desc, hlbm_domain** out) {
   if (h->d.storage != HLBM_F64) {
     int lo[3] = {0, 0, 0}, hi[3] = {h->nx - 1, h->ny - 1, h->nz - 1};
     const RestShift sh = dom_shift(h);
-    if (launch_set_distribution(h->d.storage, h->buf[0], h->stride, h->nx, h->ny, lo, hi, 0.0, sh, 0) != hipSuccess ||
-        (!aa && launch_set_distribution(h->d.storage, h->buf[1], h->stride, h->nx, h->ny, lo, hi, 0.0, sh, 0) !=
+    if (launch_set_distribution(h->d.storage, h->buf[0], h->stride, h->lay, lo, hi, 0.0, sh, 0) != hipSuccess ||
+        (!aa && launch_set_distribution(h->d.storage, h->buf[1], h->stride, h->lay, lo, hi, 0.0, sh, 0) !=
                     hipSuccess) ||
         hipDeviceSynchronize() != hipSuccess) {
       hlbm_domain_destroy(h);
@@ -649,7 +689,7 @@ int hlbm_add_obstacles(hlbm_domain* h, const hlbm_obstacle* list, int32_t n) {
   ObstacleDev* dobs = nullptr;
   HIP_TRY(hipMalloc(&dobs, obs.size() * sizeof(ObstacleDev)));
   hipError_t e = hipMemcpy(dobs, obs.data(), obs.size() * sizeof(ObstacleDev), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = launch_rasterize(h->flags, h->nx, h->ny, h->nz, dx, dobs, (int)obs.size(), 0);
+  if (e == hipSuccess) e = launch_rasterize(h->flags, h->nx, h->ny, h->nz, dx, dobs, (int)obs.size(), h->lay, 0);
   if (e == hipSuccess && h->gflags)
     e = launch_rasterize_ghost(h->gflags, h->nx, h->ny, h->nz, per, dx, dobs, (int)obs.size(), 0);
   if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -681,6 +721,11 @@ int hlbm_set_flags(hlbm_domain* h, const int32_t* obst, int32_t layout) {
         if (v != -1 && v != -2) return fail(HLBM_ERR_ARG, "obstacle values must be -1 (fluid) or -2 (wall)");
         fl[k] = v == -2 ? 1 : 0;
       }
+  if (!default_layout_p(h)) {
+    std::vector<uint8_t> lf(h->N);
+    to_layout(h, fl.data(), lf.data());
+    fl.swap(lf);
+  }
   HIP_TRY(hipMemcpy(h->flags, fl.data(), h->N, hipMemcpyHostToDevice));
   if (h->gflags) {
     const int per[3] = {h->d.periodic[0] ? 1 : 0, h->d.periodic[1] ? 1 : 0, h->d.periodic[2] ? 1 : 0};
@@ -698,7 +743,7 @@ int hlbm_set_flags(hlbm_domain* h, const int32_t* obst, int32_t layout) {
       HIP_TRY(hipMemcpy(h->gflags, g.data(), g.size(), hipMemcpyHostToDevice));
     } else {
       // node layout carries no ghosts: they take their periodic images' flags
-      HIP_TRY(launch_ghost_images(h->gflags, h->flags, h->nx, h->ny, h->nz, per, 0));
+      HIP_TRY(launch_ghost_images(h->gflags, h->flags, h->nx, h->ny, h->nz, per, h->lay, 0));
       HIP_TRY(hipDeviceSynchronize());
     }
   }
@@ -711,6 +756,11 @@ int hlbm_get_flags(hlbm_domain* h, int32_t* obst, int32_t layout) {
   if (!h || !obst) return fail(HLBM_ERR_ARG, "NULL argument");
   std::vector<uint8_t> fl(h->N);
   HIP_TRY(hipMemcpy(fl.data(), h->flags, h->N, hipMemcpyDeviceToHost));
+  if (!default_layout_p(h)) {
+    std::vector<uint8_t> nf(h->N);
+    from_layout(h, fl.data(), nf.data());
+    fl.swap(nf);
+  }
   size_t k = 0;
   for (int z = 0; z < h->nz; z++)
     for (int y = 0; y < h->ny; y++)
@@ -739,7 +789,7 @@ int hlbm_set_distribution(hlbm_domain* h, double value, const double* bounds) {
       hi[d] = std::min(b, nn[d] - 1);
     }
   }
-  HIP_TRY(launch_set_distribution(h->d.storage, h->buf[h->cur], h->stride, h->nx, h->ny, lo, hi, value, dom_shift(h),
+  HIP_TRY(launch_set_distribution(h->d.storage, h->buf[h->cur], h->stride, h->lay, lo, hi, value, dom_shift(h),
                                   0));
   HIP_TRY(hipDeviceSynchronize());
   return HLBM_OK;
@@ -755,7 +805,7 @@ int hlbm_init_double_couette(hlbm_domain* h, int32_t dim, const double U[3]) {
   double dU[3];
   for (int d = 0; d < 3; d++) dU[d] = U[d] / (0.5 * (n - 1));
   HIP_TRY(launch_double_couette(h->d.storage, h->buf[h->cur], h->stride, h->nx, h->ny, h->nz, dim, dU, U,
-                                dom_shift(h), 0));
+                                dom_shift(h), h->lay, 0));
   HIP_TRY(hipDeviceSynchronize());
   return HLBM_OK;
 }
@@ -769,7 +819,7 @@ int hlbm_set_populations_f64(hlbm_domain* h, const double* f, int32_t layout) {
   if (!h || !f) return fail(HLBM_ERR_ARG, "NULL argument");
   int rc = ensure_tmp(h);
   if (rc) return rc;
-  std::vector<double> plane;
+  std::vector<double> plane, lplane;
   const size_t Nall = ref_count(h);
   char* base = (char*)h->buf[h->cur];
   for (int i = 0; i < Q; i++) {
@@ -781,6 +831,11 @@ int hlbm_set_populations_f64(hlbm_domain* h, const double* f, int32_t layout) {
         for (int y = 0; y < h->ny; y++)
           for (int x = 0; x < h->nx; x++, k++) plane[k] = f[(size_t)i * Nall + ref_index(h, x, y, z)];
       src = plane.data();
+    }
+    if (!default_layout_p(h)) {
+      lplane.resize(h->N);
+      to_layout(h, src, lplane.data());
+      src = lplane.data();
     }
     void* dst = base + (size_t)i * h->stride * h->esize;
     if (h->d.storage == HLBM_F64) {
@@ -846,11 +901,12 @@ int hlbm_get_populations_f64(hlbm_domain* h, double* f, int32_t layout) {
   if (rc) return rc;
   rc = ensure_tmp(h);
   if (rc) return rc;
-  std::vector<double> plane(layout == HLBM_LAYOUT_REF ? h->N : 0);
+  std::vector<double> plane(layout == HLBM_LAYOUT_REF ? h->N : 0), lplane(default_layout_p(h) ? 0 : h->N);
   const size_t Nall = ref_count(h);
   char* base = (char*)h->buf[h->cur];
   for (int i = 0; i < Q; i++) {
-    double* dst = layout == HLBM_LAYOUT_REF ? plane.data() : f + (size_t)i * h->N;
+    double* out = layout == HLBM_LAYOUT_REF ? plane.data() : f + (size_t)i * h->N;
+    double* dst = default_layout_p(h) ? out : lplane.data();
     void* srcp = base + (size_t)i * h->stride * h->esize;
     if (h->d.storage == HLBM_F64) {
       HIP_TRY(hipMemcpy(dst, srcp, (size_t)h->N * 8, hipMemcpyDeviceToHost));
@@ -858,6 +914,7 @@ int hlbm_get_populations_f64(hlbm_domain* h, double* f, int32_t layout) {
       HIP_TRY(launch_convert(h->d.storage, false, srcp, h->tmp, h->N, i, dom_shift(h), 0));
       HIP_TRY(hipMemcpy(dst, h->tmp, (size_t)h->N * 8, hipMemcpyDeviceToHost));
     }
+    if (!default_layout_p(h)) from_layout(h, lplane.data(), out);
     if (layout == HLBM_LAYOUT_REF) {
       size_t k = 0;
       for (int z = 0; z < h->nz; z++)
@@ -916,6 +973,15 @@ static int copy_macro_out(hlbm_domain* h, double* rho, double* u, int32_t layout
   std::vector<double> r(N), v(3 * N);
   HIP_TRY(hipMemcpy(r.data(), h->m0, N * 8, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(v.data(), h->m1, 3 * N * 8, hipMemcpyDeviceToHost));
+  if (!default_layout_p(h)) {
+    std::vector<double> t(N);
+    from_layout(h, r.data(), t.data());
+    r.swap(t);
+    for (int c = 0; c < 3; c++) {
+      from_layout(h, v.data() + c * N, t.data());
+      std::memcpy(v.data() + c * N, t.data(), N * 8);
+    }
+  }
   if (layout != HLBM_LAYOUT_REF) {
     if (rho) std::memcpy(rho, r.data(), N * 8);
     if (u) std::memcpy(u, v.data(), 3 * N * 8);
@@ -966,7 +1032,8 @@ int hlbm_plane_profile(hlbm_domain* h, int32_t dim, double scale, double* sum, i
   ProfileStat* dev = nullptr;
   HIP_TRY(hipMalloc((void**)&dev, (parts + (size_t)n[dim]) * sizeof(ProfileStat)));
   std::vector<ProfileStat> host(n[dim]);
-  hipError_t e = launch_plane_profile(h->m1, h->flags, h->N, h->nx, h->ny, h->nz, dim, scale, dev, dev + parts, 0);
+  hipError_t e =
+      launch_plane_profile(h->m1, h->flags, h->N, h->nx, h->ny, h->nz, dim, scale, dev, dev + parts, h->lay, 0);
   if (e == hipSuccess)
     e = hipMemcpy(host.data(), dev + parts, (size_t)n[dim] * sizeof(ProfileStat), hipMemcpyDeviceToHost);
   (void)hipFree(dev);
@@ -987,7 +1054,7 @@ int hlbm_plane_profile_noalloc(hlbm_domain* h, int32_t dim, double scale, hlbm_p
     return fail(HLBM_ERR_ARG, "plane profile: more columns than HLBM_PROFILE_SCRATCH_COLUMNS");
   ProfileStat* part = nullptr;
   HIP_TRY(profile_scratch(&part));
-  HIP_TRY(launch_plane_profile(h->m1, h->flags, h->N, h->nx, h->ny, h->nz, dim, scale, part, (ProfileStat*)out,
+  HIP_TRY(launch_plane_profile(h->m1, h->flags, h->N, h->nx, h->ny, h->nz, dim, scale, part, (ProfileStat*)out, h->lay,
                                (hipStream_t)stream));
   return HLBM_OK;
 }
@@ -998,7 +1065,7 @@ int hlbm_line_velocity(hlbm_domain* h, const int32_t start[3], int32_t axis, int
   double* dev = nullptr;
   HIP_TRY(hipMalloc((void**)&dev, (size_t)n * 3 * 8));
   const int st[3] = {start[0], start[1], start[2]};
-  hipError_t e = launch_line_velocity(h->m1, h->N, h->nx, h->ny, h->nz, st, axis, n, scale, dev, 0);
+  hipError_t e = launch_line_velocity(h->m1, h->N, h->nx, h->ny, h->nz, st, axis, n, scale, dev, h->lay, 0);
   if (e == hipSuccess) e = hipMemcpy(u, dev, (size_t)n * 3 * 8, hipMemcpyDeviceToHost);
   (void)hipFree(dev);
   if (e != hipSuccess) return fail(HLBM_ERR_HIP, hipGetErrorString(e));
@@ -1046,6 +1113,49 @@ int hlbm_set_launch(hlbm_domain* h, int32_t block_x, int32_t block_y, int32_t np
   return HLBM_OK;
 }
 
+int hlbm_set_layout(hlbm_domain* h, const int32_t order[3]) {
+  if (!h || !order) return fail(HLBM_ERR_ARG, "NULL argument");
+  const int o[3] = {order[0], order[1], order[2]};
+  bool seen[3] = {false, false, false};
+  for (int k = 0; k < 3; k++) {
+    if (o[k] < 0 || o[k] > 2 || seen[o[k]]) return fail(HLBM_ERR_ARG, "order must be a permutation of 0, 1, 2");
+    seen[o[k]] = true;
+  }
+  if (o[0] == h->lay.ord[0] && o[1] == h->lay.ord[1]) return HLBM_OK;
+  // the pending streaming belongs to the old layout; the periodic-face wall list
+  // and the bounce-back mask are rebuilt for the new one
+  {
+    const int rc = materialize_for_change(h);
+    if (rc) return rc;
+  }
+  {
+    const int rc = ensure_tmp(h);
+    if (rc) return rc;
+  }
+  const int n[3] = {h->nx, h->ny, h->nz};
+  const NodeLayout from = h->lay, to = make_layout(n, o);
+  // every per-node array through the FP64 scratch plane, one plane at a time
+  auto move = [&](void* a, size_t esize) -> hipError_t {
+    hipError_t e = launch_permute(h->tmp, a, esize, n, from, to, 0);
+    if (e == hipSuccess) e = hipMemcpyAsync(a, h->tmp, (size_t)h->N * esize, hipMemcpyDeviceToDevice, 0);
+    return e;
+  };
+  for (int i = 0; i < Q; i++) HIP_TRY(move((char*)h->buf[h->cur] + (size_t)i * h->stride * h->esize, h->esize));
+  HIP_TRY(move(h->flags, 1));
+  HIP_TRY(move(h->m0, 8));
+  for (int c = 0; c < 3; c++) HIP_TRY(move(h->m1 + (size_t)c * h->N, 8));
+  HIP_TRY(hipDeviceSynchronize());
+  h->lay = to;
+  h->bbm_dirty = true;
+  return HLBM_OK;
+}
+
+int hlbm_get_layout(const hlbm_domain* h, int32_t order[3]) {
+  if (!h || !order) return fail(HLBM_ERR_ARG, "NULL argument");
+  for (int k = 0; k < 3; k++) order[k] = h->lay.ord[k];
+  return HLBM_OK;
+}
+
 namespace {
 struct TuneKey {
   int nx, ny, nz, storage, collision, pattern, prog_planes, wall_bb, device;
@@ -1058,6 +1168,7 @@ struct TuneKey {
 struct TuneVal {
   int bx, by, npt, remap, variant;
   double ms_best, ms_default;
+  int order[3];
 };
 std::mutex tune_mu;
 std::map<TuneKey, TuneVal> tune_cache;
@@ -1188,6 +1299,72 @@ static int tune_placement(hlbm_domain* h, hipStream_t s, hipEvent_t e0, hipEvent
   return rc;
 }
 
+static int pack_order(const hlbm_domain* h) { return 100 * h->lay.ord[0] + 10 * h->lay.ord[1] + h->lay.ord[2]; }
+
+// Node layouts worth trying: the current one, and for every axis that holds
+// planes of the boundary program, the one that puts that axis last (its planes
+// become contiguous); the fastest axis must fill at least half a wave.
+static std::vector<std::array<int, 3>> layout_candidates(const hlbm_domain* h) {
+  std::vector<std::array<int, 3>> c{{h->lay.ord[0], h->lay.ord[1], h->lay.ord[2]}};
+  const int n[3] = {h->nx, h->ny, h->nz};
+  for (int a = 0; a < 3; a++) {
+    const int bits = a == 2 ? 3 : (a == 1 ? 12 : 48);
+    if (!(h->plan.prog_planes & bits)) continue;
+    std::array<int, 3> o{};
+    int k = 0;
+    for (int b = 0; b < 3; b++)
+      if (b != a) o[k++] = b;
+    o[2] = a;
+    if (n[o[0]] < 32) continue;
+    if (std::find(c.begin(), c.end(), o) == c.end()) c.push_back(o);
+  }
+  return c;
+}
+
+// Time each candidate layout with the default launch shape and keep the fastest.
+static int choose_layout(hlbm_domain* h, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  const auto cand = layout_candidates(h);
+  if (cand.size() < 2) return HLBM_OK;
+  const dim3 keep_block = h->block;
+  const int keep_npt = h->npt, keep_remap = h->xcd_remap, keep_variant = h->variant;
+  hlbm_set_launch(h, 128, 2, 1, 1, 0);
+  const bool aa = h->d.pattern == HLBM_PATTERN_AA;
+  const size_t fbytes = (size_t)h->stride * Q * h->esize;
+  double best = 1e300;
+  std::array<int, 3> pick = cand[0];
+  int rc = HLBM_OK;
+  for (const auto& o : cand) {
+    rc = hlbm_set_layout(h, o.data());
+    if (rc) break;
+    void* scratch = nullptr;  // AA steps work in place: time them on a copy
+    if (aa) {
+      if (hipMalloc(&scratch, fbytes) != hipSuccess) {
+        rc = fail(HLBM_ERR_NOMEM, "autotune scratch populations");
+        break;
+      }
+      if (hipMemcpyAsync(scratch, h->buf[h->cur], fbytes, hipMemcpyDeviceToDevice, s) != hipSuccess) {
+        (void)hipFree(scratch);
+        rc = fail(HLBM_ERR_HIP, "autotune scratch copy");
+        break;
+      }
+    }
+    double ms = 0.0;
+    rc = time_shape(h, scratch, reps, s, e0, e1, &ms);
+    if (scratch) (void)hipFree(scratch);
+    if (rc) break;
+    if (ms < best) {
+      best = ms;
+      pick = o;
+    }
+  }
+  if (rc == HLBM_OK) rc = hlbm_set_layout(h, pick.data());
+  h->block = keep_block;
+  h->npt = keep_npt;
+  h->xcd_remap = keep_remap;
+  h->variant = keep_variant;
+  return rc;
+}
+
 int hlbm_autotune(hlbm_domain* h, int32_t reps, void* stream, hlbm_tune_info* out) {
   if (!h) return fail(HLBM_ERR_ARG, "NULL domain");
   if (reps < 1) reps = 1;
@@ -1223,11 +1400,18 @@ int hlbm_autotune(hlbm_domain* h, int32_t reps, void* stream, hlbm_tune_info* ou
     std::lock_guard<std::mutex> g(tune_mu);
     auto it = tune_cache.find(key);
     if (it != tune_cache.end()) {
-      const TuneVal& v = it->second;
+      const TuneVal v = it->second;
+      const int rc = hlbm_set_layout(h, v.order);
+      if (rc) return done(rc);
       hlbm_set_launch(h, v.bx, v.by, v.npt, v.remap, v.variant);
-      if (out) *out = hlbm_tune_info{v.bx, v.by, v.npt, v.remap, 0, v.variant, moved, 0, v.ms_best, v.ms_default};
+      if (out)
+        *out = hlbm_tune_info{v.bx, v.by, v.npt, v.remap, 0, v.variant, moved, pack_order(h), v.ms_best, v.ms_default};
       return done(HLBM_OK);
     }
+  }
+  {
+    const int rc = choose_layout(h, reps, s, e0, e1);
+    if (rc) return done(rc);
   }
   void* scratch = nullptr;
   if (h->d.pattern == HLBM_PATTERN_AA) {
@@ -1241,7 +1425,7 @@ int hlbm_autotune(hlbm_domain* h, int32_t reps, void* stream, hlbm_tune_info* ou
   const dim3 keep_block = h->block;
   const int keep_npt = h->npt, keep_remap = h->xcd_remap, keep_variant = h->variant;
   static const int shapes[][2] = {{128, 2}, {64, 1}, {64, 2}, {64, 4}, {128, 1}, {256, 1}};
-  TuneVal best{128, 2, 1, 1, 0, 1e300, 0.0};
+  TuneVal best{128, 2, 1, 1, 0, 1e300, 0.0, {h->lay.ord[0], h->lay.ord[1], h->lay.ord[2]}};
   int rc = HLBM_OK, n = 0;
   {
     // short steps: time enough of them that each candidate's measurement spans
@@ -1252,7 +1436,8 @@ int hlbm_autotune(hlbm_domain* h, int32_t reps, void* stream, hlbm_tune_info* ou
   }
   for (int variant = 0; variant <= 1 && rc == HLBM_OK; variant++) {
     for (const auto& sh : shapes) {
-      for (int npt = 1; npt <= (variant ? 1 : 2) && rc == HLBM_OK; npt++) {
+      // a layout other than the default runs one node per thread
+      for (int npt = 1; npt <= (variant || !default_layout_p(h) ? 1 : 2) && rc == HLBM_OK; npt++) {
         for (int remap = 1; remap >= 0 && rc == HLBM_OK; remap--) {
           hlbm_set_launch(h, sh[0], sh[1], npt, remap, variant);
           double ms = 0.0;
@@ -1260,7 +1445,9 @@ int hlbm_autotune(hlbm_domain* h, int32_t reps, void* stream, hlbm_tune_info* ou
           if (rc) break;
           n++;
           if (sh[0] == 128 && sh[1] == 2 && npt == 1 && remap == 1 && variant == 0) best.ms_default = ms;
-          if (ms < best.ms_best) best = TuneVal{sh[0], sh[1], npt, remap, variant, ms, best.ms_default};
+          if (ms < best.ms_best)
+            best = TuneVal{sh[0], sh[1], npt, remap, variant, ms, best.ms_default, {best.order[0], best.order[1],
+                                                                                   best.order[2]}};
         }
       }
       if (rc) break;
@@ -1281,7 +1468,7 @@ int hlbm_autotune(hlbm_domain* h, int32_t reps, void* stream, hlbm_tune_info* ou
     std::lock_guard<std::mutex> g(tune_mu);
     tune_cache[key] = best;
   }
-  if (out) *out = hlbm_tune_info{best.bx, best.by, best.npt, best.remap, n, best.variant, moved, 0, best.ms_best,
+  if (out) *out = hlbm_tune_info{best.bx, best.by, best.npt, best.remap, n, best.variant, moved, pack_order(h), best.ms_best,
                                  best.ms_default};
   return HLBM_OK;
 }

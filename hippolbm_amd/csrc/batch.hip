@@ -159,6 +159,10 @@ static StepParams single_params(hlbm_batch* b, const Solve& s, int parity) {
   for (int k = 0; k < 3; k++) p.per[k] = s.dom.per[k];
   p.N = s.N;
   p.stride = s.stride;
+  p.lay = default_layout(s.dom.nx, s.dom.ny, s.dom.nz);  // batch solves keep the default layout
+  p.ln[0] = s.dom.nx;
+  p.ln[1] = s.dom.ny;
+  p.ln[2] = s.dom.nz;
   p.fin = (const char*)b->arena[parity] + s.f_off * b->esize;
   p.fout = (char*)b->arena[parity ^ 1] + s.f_off * b->esize;
   p.flags = b->flags + s.flag_off;
@@ -453,17 +457,21 @@ static int admit_one(hlbm_batch* b, Solve& s) {
       std::memcpy(hs, od.data(), od.size() * sizeof(ObstacleDev));
       b->obs_used[set] += od.size();
       hipError_t e = hipMemcpyAsync(ds, hs, od.size() * sizeof(ObstacleDev), hipMemcpyHostToDevice, st);
-      if (e == hipSuccess) e = launch_rasterize(b->flags + s.flag_off, s.dom.nx, s.dom.ny, s.dom.nz, dx, ds, (int)od.size(), st);
+      if (e == hipSuccess)
+        e = launch_rasterize(b->flags + s.flag_off, s.dom.nx, s.dom.ny, s.dom.nz, dx, ds, (int)od.size(),
+                             default_layout(s.dom.nx, s.dom.ny, s.dom.nz), st);
       if (e != hipSuccess) return bfail(HLBM_ERR_HIP, "rasterize");
     }
   }
-  if (launch_build_bbm(b->flags + s.flag_off, s.dom.nx, s.dom.ny, s.dom.nz, s.dom.per, b->bbm + s.flag_off, st) !=
+  if (launch_build_bbm(b->flags + s.flag_off, s.dom.nx, s.dom.ny, s.dom.nz, s.dom.per, b->bbm + s.flag_off,
+                       default_layout(s.dom.nx, s.dom.ny, s.dom.nz), st) !=
       hipSuccess)
     return bfail(HLBM_ERR_HIP, "bounce-back mask");
   void* fa = (char*)b->arena[0] + s.f_off * b->esize;
   if (s.init_f.empty()) {
     int lo[3] = {0, 0, 0}, hi[3] = {s.dom.nx - 1, s.dom.ny - 1, s.dom.nz - 1};
-    if (launch_set_distribution(b->d.storage, fa, s.stride, s.dom.nx, s.dom.ny, lo, hi, s.desc.init_rho,
+    if (launch_set_distribution(b->d.storage, fa, s.stride, default_layout(s.dom.nx, s.dom.ny, s.dom.nz), lo, hi,
+                                s.desc.init_rho,
                                 solve_shift(s), st) !=
         hipSuccess)
       return bfail(HLBM_ERR_HIP, "set_distribution");

@@ -73,6 +73,32 @@ inline RestShift rest_shift(int mrt, int mrt_custom, const double* rates) {
   return r;
 }
 
+// Node layout in HBM: node (x, y, z) at x*st[0] + y*st[1] + z*st[2].  The axes
+// are ordered fastest first by `ord`; the default is x fastest, then y, then z
+// (box3d.hpp:127-132).  Other orders put the planes of a chosen axis in
+// contiguous memory (e.g. x slowest for x-normal boundary planes).
+struct NodeLayout {
+  uint32_t st[3];
+  int32_t ord[3];
+};
+inline NodeLayout make_layout(const int n[3], const int ord[3]) {
+  NodeLayout L;
+  uint32_t s = 1;
+  for (int k = 0; k < 3; k++) {
+    L.ord[k] = ord[k];
+    L.st[ord[k]] = s;
+    s *= (uint32_t)n[ord[k]];
+  }
+  return L;
+}
+inline NodeLayout default_layout(int nx, int ny, int nz) {
+  const int n[3] = {nx, ny, nz}, ord[3] = {0, 1, 2};
+  return make_layout(n, ord);
+}
+HLBM_HD inline uint32_t node_at(const NodeLayout& L, int x, int y, int z) {
+  return (uint32_t)x * L.st[0] + (uint32_t)y * L.st[1] + (uint32_t)z * L.st[2];
+}
+
 // plane index (hlbm_plane) -> (dim, side)
 HLBM_HD constexpr int plane_dim(int p) { return p < 2 ? 2 : (p < 4 ? 1 : 0); }
 HLBM_HD constexpr int plane_side(int p) { return p & 1; }

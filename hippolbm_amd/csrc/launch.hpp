@@ -51,7 +51,7 @@ inline unsigned boundary_blocks(const StepParams& p) {
 // special_kernel (nodes next to a wall across a periodic face).  The
 // program-plane nodes, the special nodes and the bulk touch disjoint nodes
 // (A/B) or disjoint slots (AA), so their order is free.
-template <class S, int COLL, int MODE, int NPT, bool WIDE>
+template <class S, int COLL, int MODE, int NPT, bool WIDE, bool PERM>
 hipError_t step_go(dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
   hipError_t e = hipSuccess;
   if constexpr (MODE == MODE_AA_EVEN || MODE == MODE_AA_ODD) {
@@ -68,7 +68,7 @@ hipError_t step_go(dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
     if (e != hipSuccess) return e;
   }
   if (MODE != MODE_AA_FIX) {  // AA_FIX only touches program-plane nodes
-    hipLaunchKernelGGL((step_kernel<S, COLL, MODE, NPT, WIDE>), grid, block, 0, s, p);
+    hipLaunchKernelGGL((step_kernel<S, COLL, MODE, NPT, WIDE, PERM>), grid, block, 0, s, p);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -82,11 +82,16 @@ hipError_t step_go(dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
   return hipGetLastError();
 }
 
+// A layout other than the default runs one node per thread (npt 1).
 template <class S, int COLL, int MODE>
 hipError_t step_go_npt(int npt, dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
-  if (wide_offsets(p, sizeof(S))) return step_go<S, COLL, MODE, 1, true>(grid, block, p, s);
-  return npt == 2 ? step_go<S, COLL, MODE, 2, false>(grid, block, p, s)
-                  : step_go<S, COLL, MODE, 1, false>(grid, block, p, s);
+  const bool perm = !(p.lay.ord[0] == 0 && p.lay.ord[1] == 1);
+  if (perm)
+    return wide_offsets(p, sizeof(S)) ? step_go<S, COLL, MODE, 1, true, true>(grid, block, p, s)
+                                      : step_go<S, COLL, MODE, 1, false, true>(grid, block, p, s);
+  if (wide_offsets(p, sizeof(S))) return step_go<S, COLL, MODE, 1, true, false>(grid, block, p, s);
+  return npt == 2 ? step_go<S, COLL, MODE, 2, false, false>(grid, block, p, s)
+                  : step_go<S, COLL, MODE, 1, false, false>(grid, block, p, s);
 }
 
 template <class S>

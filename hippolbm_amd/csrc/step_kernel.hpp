@@ -72,6 +72,8 @@ struct StepParams {
   uint32_t N;           // nodes
   uint32_t pad0;
   uint64_t stride;      // elements between two population planes
+  NodeLayout lay;       // node (x, y, z) at lay.st . (x, y, z); lay.ord: the axes, fastest first
+  int32_t ln[3];        // nodes along lay.ord[0..2] (the bulk kernel's lane, row and grid axes)
   const void* fin;
   void* fout;
   const uint8_t* flags; // 0 fluid, 1 wall
@@ -717,6 +719,26 @@ __device__ __forceinline__ uint32_t plane_slot(int pl, int x, int y, int z, int 
   return b * na + a;
 }
 
+__device__ __forceinline__ uint32_t node_index(const StepParams& p, int x, int y, int z) {
+  return node_at(p.lay, x, y, z);
+}
+
+// The node's index in the default layout (x fastest): stochastic rounding is
+// keyed on it, so reduced-precision results do not depend on the layout.
+__device__ __forceinline__ uint32_t canonical_index(const StepParams& p, int x, int y, int z) {
+  return ((uint32_t)z * (uint32_t)p.ny + (uint32_t)y) * (uint32_t)p.nx + (uint32_t)x;
+}
+
+// The coordinates of a node from its index (lay.ord picks which of the three
+// layout coordinates is x, y and z; the choices are uniform selects).
+__device__ __forceinline__ void node_coords(const StepParams& p, uint32_t idx, int& x, int& y, int& z) {
+  const uint32_t n0 = (uint32_t)p.ln[0], n01 = n0 * (uint32_t)p.ln[1];
+  const uint32_t c2 = idx / n01, r = idx - c2 * n01, c1 = r / n0, c0 = r - c1 * n0;
+  x = (int)(p.lay.ord[0] == 0 ? c0 : (p.lay.ord[1] == 0 ? c1 : c2));
+  y = (int)(p.lay.ord[0] == 1 ? c0 : (p.lay.ord[1] == 1 ? c1 : c2));
+  z = (int)(p.lay.ord[0] == 2 ? c0 : (p.lay.ord[1] == 2 ? c1 : c2));
+}
+
 __device__ __forceinline__ int node_planes(int x, int y, int z, int nx, int ny, int nz) {
   return (z == 0 ? 1 : 0) | (z == nz - 1 ? 2 : 0) | (y == 0 ? 4 : 0) | (y == ny - 1 ? 8 : 0) | (x == 0 ? 16 : 0) |
          (x == nx - 1 ? 32 : 0);
@@ -757,7 +779,7 @@ __device__ __forceinline__ typename StoreOps<S>::C pull_one(const StepParams& p,
       }
       return v;
     }
-    const uint32_t sidx = ((uint32_t)sz * (uint32_t)ny + (uint32_t)sy) * (uint32_t)nx + (uint32_t)sx;
+    const uint32_t sidx = node_index(p, sx, sy, sz);
     if (I != 0 && p.wall_bb && !wall && p.flags[sidx] != 0) return O::ld(fin, (size_t)(AA ? I : OPP(I)) * st + idx);
     return O::ld(fin, (size_t)(AA ? OPP(I) : I) * st + sidx);
   }
@@ -836,7 +858,7 @@ __device__ __forceinline__ void apply_program(const StepParams& p, int x, int y,
               } else {
                 const int ax = x + (dim == 0 ? inward : 0), ay = y + (dim == 1 ? inward : 0),
                           az = z + (dim == 2 ? inward : 0);
-                const uint32_t aidx = ((uint32_t)az * (uint32_t)ny + (uint32_t)ay) * (uint32_t)nx + (uint32_t)ax;
+                const uint32_t aidx = node_index(p, ax, ay, az);
                 const bool awall = p.flags[aidx] != 0;
                 static_for<1, Q>([&](auto ic) {
                   constexpr int i = decltype(ic)::value;
@@ -922,7 +944,7 @@ __device__ __forceinline__ void store_node(const StepParams& p, int x, int y, in
   using C = typename O::C;
   S* __restrict__ fout = (S*)p.fout;
   const size_t st = p.stride;
-  const Rounder<S> R(p, idx);
+  const Rounder<S> R(p, StoreOps<S>::narrow16 ? canonical_index(p, x, y, z) : idx);
   if constexpr (MODE == MODE_AA_EVEN) {
 #pragma unroll
     for (int i = 0; i < Q; i++) O::st_sr(fout, (size_t)OPP(i) * st + idx, f[i], R.word(i));
@@ -951,7 +973,7 @@ __device__ __forceinline__ void store_node(const StepParams& p, int x, int y, in
         }
         O::st_sr(fout, (size_t)OPP(i) * st + idx, v, R.word(i));
       } else {
-        const uint32_t tidx = ((uint32_t)tz * (uint32_t)ny + (uint32_t)ty) * (uint32_t)nx + (uint32_t)tx;
+        const uint32_t tidx = node_index(p, tx, ty, tz);
         if (i != 0 && wbb && p.flags[tidx] != 0)
           O::st_sr(fout, (size_t)OPP(i) * st + idx, f[i], R.word(i));
         else
@@ -1048,8 +1070,8 @@ __device__ __forceinline__ NodeNbrs node_nbrs(const StepParams& p, int x, int y,
   NodeNbrs a;
   const int c[3] = {x, y, z};
   const int n[3] = {p.nx, p.ny, p.nz};
-  const uint32_t unit[3] = {(uint32_t)sizeof(S), (uint32_t)sizeof(S) * (uint32_t)p.nx,
-                            (uint32_t)sizeof(S) * (uint32_t)p.nx * (uint32_t)p.ny};
+  const uint32_t unit[3] = {(uint32_t)sizeof(S) * p.lay.st[0], (uint32_t)sizeof(S) * p.lay.st[1],
+                            (uint32_t)sizeof(S) * p.lay.st[2]};
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     const bool lo = c[d] == 0, hi = c[d] == n[d] - 1;
@@ -1172,14 +1194,14 @@ __device__ __forceinline__ void gather_bulk(const StepParams& p, uint32_t bo, co
 
 // Store for a bulk node (same result as store_node).
 template <class S, int MODE>
-__device__ __forceinline__ void store_bulk(const StepParams& p, uint32_t bo, const NodeNbrs& a, uint32_t wbits,
-                                           typename StoreOps<S>::C* f) {
+__device__ __forceinline__ void store_bulk(const StepParams& p, uint32_t bo, uint32_t cidx, const NodeNbrs& a,
+                                           uint32_t wbits, typename StoreOps<S>::C* f) {
   using O = StoreOps<S>;
   using C = typename O::C;
   S* __restrict__ fout = (S*)p.fout;
   const size_t st = p.stride;
   S* b = fout;
-  const Rounder<S> R(p, bo / (uint32_t)sizeof(S));
+  const Rounder<S> R(p, cidx);
   bo = lane_offset(bo);
   if constexpr (MODE == MODE_AA_EVEN) {
     static_for([&](auto jc) {
@@ -1228,7 +1250,7 @@ __device__ __forceinline__ void store_bulk(const StepParams& p, uint32_t bo, con
 template <class S, int COLL, int MODE, bool PROG>
 __device__ __forceinline__ void process_node(const StepParams& p, int x, int y, int z) {
   using C = typename StoreOps<S>::C;
-  const uint32_t idx = ((uint32_t)z * (uint32_t)p.ny + (uint32_t)y) * (uint32_t)p.nx + (uint32_t)x;
+  const uint32_t idx = node_index(p, x, y, z);
   const bool wall = p.flags[idx] != 0;
   if (mode_skips_walls(MODE) && wall && p.wall_bb) return;
   if (p.nspec && (p.bbm[idx] >> 31)) return;  // special_kernel's node
@@ -1244,7 +1266,9 @@ __device__ __forceinline__ void process_node(const StepParams& p, int x, int y, 
 // A tile is blockDim.x nodes along x by blockDim.y*NPT rows along y; each
 // thread gathers its NPT nodes first (19*NPT loads in flight), then collides
 // and stores them.
-template <class S, int COLL, int MODE, int NPT, bool WIDE>
+// PERM: the node layout is not the default one (x fastest); the default layout
+// compiles the index arithmetic down to fixed axes.
+template <class S, int COLL, int MODE, int NPT, bool WIDE, bool PERM>
 __global__ void __launch_bounds__(256) step_kernel(const StepParams p) {
   using C = typename StoreOps<S>::C;
   // 1-D dispatch over tiles.  Workgroups are handed to the 8 XCDs round-robin;
@@ -1256,23 +1280,47 @@ __global__ void __launch_bounds__(256) step_kernel(const StepParams p) {
     const uint32_t per = nt / 8u;
     if (t < per * 8u) t = (t % 8u) * per + t / 8u;
   }
+  // lanes run along the layout's fastest axis, tile rows along the next one and
+  // the grid along the slowest: l, row and c2 below
   const int bx = (int)(t % (uint32_t)p.gx);
   const uint32_t r = t / (uint32_t)p.gx;
   const int by = (int)(r % (uint32_t)p.gy);
-  const int z = (int)(r / (uint32_t)p.gy);
-  const int x = bx * blockDim.x + threadIdx.x;
-  if (x >= p.nx || z >= p.nz) return;
+  const int c2 = (int)(r / (uint32_t)p.gy);
+  const int l = bx * blockDim.x + threadIdx.x;
+  if (l >= p.ln[0] || c2 >= p.ln[2]) return;
+  const int o0 = PERM ? p.lay.ord[0] : 0, o1 = PERM ? p.lay.ord[1] : 1;
+  auto coords = [&](int row, int& x, int& y, int& z) {
+    if constexpr (PERM) {
+      x = o0 == 0 ? l : (o1 == 0 ? row : c2);
+      y = o0 == 1 ? l : (o1 == 1 ? row : c2);
+      z = o0 == 2 ? l : (o1 == 2 ? row : c2);
+    } else {
+      x = l;
+      y = row;
+      z = c2;
+    }
+  };
+  // the default layout: the node index and the stochastic-rounding key coincide
+  auto index_of = [&](int x, int y, int z) -> uint32_t {
+    if constexpr (PERM)
+      return node_index(p, x, y, z);
+    else
+      return ((uint32_t)z * (uint32_t)p.ny + (uint32_t)y) * (uint32_t)p.nx + (uint32_t)x;
+  };
   if constexpr (!WIDE) {
     C f[NPT][Q];
-    uint32_t bo[NPT], wbits[NPT];
+    uint32_t bo[NPT], wbits[NPT], cidx[NPT];
     NodeNbrs a[NPT];
     bool wall[NPT], act[NPT];
 #pragma unroll
     for (int k = 0; k < NPT; k++) {
-      const int y = (by * NPT + k) * blockDim.y + threadIdx.y;
-      act[k] = y < p.ny && !(MODE != MODE_COLLIDE && (node_planes(x, y, z, p.nx, p.ny, p.nz) & p.prog_planes));
-      const uint32_t idx = ((uint32_t)z * (uint32_t)p.ny + (uint32_t)y) * (uint32_t)p.nx + (uint32_t)x;
+      const int row = (by * NPT + k) * blockDim.y + threadIdx.y;
+      int x, y, z;
+      coords(row, x, y, z);
+      act[k] = row < p.ln[1] && !(MODE != MODE_COLLIDE && (node_planes(x, y, z, p.nx, p.ny, p.nz) & p.prog_planes));
+      const uint32_t idx = index_of(x, y, z);
       bo[k] = idx * (uint32_t)sizeof(S);
+      cidx[k] = PERM && StoreOps<S>::narrow16 ? canonical_index(p, x, y, z) : idx;
       wbits[k] = 0u;
       wall[k] = false;
       if (act[k] && p.any_wall) {
@@ -1294,18 +1342,23 @@ __global__ void __launch_bounds__(256) step_kernel(const StepParams p) {
     for (int k = 0; k < NPT; k++) {
       if (act[k]) {
         collide_node<S, COLL, MODE>(p, bo[k] / (uint32_t)sizeof(S), wall[k], f[k]);
-        store_bulk<S, MODE>(p, bo[k], a[k], wbits[k], f[k]);
+        store_bulk<S, MODE>(p, bo[k], cidx[k], a[k], wbits[k], f[k]);
       }
     }
   } else {
   C f[NPT][Q];
   uint32_t idx[NPT];
+  int xs[NPT], ys[NPT], zs[NPT];
   bool wall[NPT], act[NPT];
 #pragma unroll
   for (int k = 0; k < NPT; k++) {
-    const int y = (by * NPT + k) * blockDim.y + threadIdx.y;
-    act[k] = y < p.ny && !(MODE != MODE_COLLIDE && (node_planes(x, y, z, p.nx, p.ny, p.nz) & p.prog_planes));
-    idx[k] = ((uint32_t)z * (uint32_t)p.ny + (uint32_t)y) * (uint32_t)p.nx + (uint32_t)x;
+    const int row = (by * NPT + k) * blockDim.y + threadIdx.y;
+    int& x = xs[k];
+    int& y = ys[k];
+    int& z = zs[k];
+    coords(row, x, y, z);
+    act[k] = row < p.ln[1] && !(MODE != MODE_COLLIDE && (node_planes(x, y, z, p.nx, p.ny, p.nz) & p.prog_planes));
+    idx[k] = index_of(x, y, z);
     if (act[k]) {
       wall[k] = p.flags[idx[k]] != 0;
       if (mode_skips_walls(MODE) && wall[k] && p.wall_bb) act[k] = false;
@@ -1315,7 +1368,7 @@ __global__ void __launch_bounds__(256) step_kernel(const StepParams p) {
   }
 #pragma unroll
   for (int k = 0; k < NPT; k++)
-    if (act[k]) finish_node<S, COLL, MODE>(p, x, (by * NPT + k) * blockDim.y + threadIdx.y, z, idx[k], wall[k], f[k]);
+    if (act[k]) finish_node<S, COLL, MODE>(p, xs[k], ys[k], zs[k], idx[k], wall[k], f[k]);
   }
 }
 
@@ -1325,18 +1378,25 @@ template <class S, int COLL, int MODE>
 __global__ void __launch_bounds__(256) boundary_kernel(const StepParams p) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   const int nx = p.nx, ny = p.ny, nz = p.nz;
+  const int n[3] = {nx, ny, nz};
   for (int pl = 0; pl < 6; pl++) {
     if (!((p.prog_planes >> pl) & 1)) continue;
     const int dim = plane_dim(pl);
-    const uint32_t na = dim == 0 ? (uint32_t)ny : (uint32_t)nx;  // fastest in-plane axis
-    const uint32_t nb = dim == 2 ? (uint32_t)ny : (uint32_t)nz;
+    // lanes along the in-plane axis that is faster in memory
+    int da = dim == 0 ? 1 : 0, db = dim == 2 ? 1 : 2;
+    if (p.lay.st[db] < p.lay.st[da]) {
+      const int sw = da;
+      da = db;
+      db = sw;
+    }
+    const uint32_t na = (uint32_t)n[da], nb = (uint32_t)n[db];
     const uint32_t cnt = na * nb;
     if (t < cnt) {
       const int a = (int)(t % na), b = (int)(t / na);
-      const int c = plane_side(pl) ? (dim == 0 ? nx - 1 : (dim == 1 ? ny - 1 : nz - 1)) : 0;
-      const int x = dim == 0 ? c : a;
-      const int y = dim == 0 ? a : (dim == 1 ? c : b);
-      const int z = dim == 2 ? c : b;
+      const int c = plane_side(pl) ? n[dim] - 1 : 0;
+      const int x = dim == 0 ? c : (da == 0 ? a : b);
+      const int y = dim == 1 ? c : (da == 1 ? a : b);
+      const int z = dim == 2 ? c : (da == 2 ? a : b);
       const int lower = node_planes(x, y, z, nx, ny, nz) & p.prog_planes & ((1 << pl) - 1);
       if (!lower) process_node<S, COLL, MODE, true>(p, x, y, z);
       return;
@@ -1365,10 +1425,8 @@ __global__ void __launch_bounds__(64) special_kernel(const StepParams p) {
   if (k >= p.nspec) return;
   const uint32_t idx = p.spec_idx[k];
   const uint32_t mask = p.spec_mask[k];
-  const uint32_t nxy = (uint32_t)p.nx * (uint32_t)p.ny;
-  const int z = (int)(idx / nxy);
-  const uint32_t r = idx - (uint32_t)z * nxy;
-  const int y = (int)(r / (uint32_t)p.nx), x = (int)(r % (uint32_t)p.nx);
+  int x, y, z;
+  node_coords(p, idx, x, y, z);
   C* sv = (C*)p.spec_val;
   const size_t ns = p.nspec;
   C f[Q];
@@ -1428,7 +1486,7 @@ __global__ void __launch_bounds__(256) outflow_capture_kernel(const StepParams p
       const int y = dim == 0 ? a : (dim == 1 ? c : b);
       const int z = dim == 2 ? c : b;
       const int ax = x + (dim == 0 ? inward : 0), ay = y + (dim == 1 ? inward : 0), az = z + (dim == 2 ? inward : 0);
-      const uint32_t aidx = ((uint32_t)az * (uint32_t)ny + (uint32_t)ay) * (uint32_t)nx + (uint32_t)ax;
+      const uint32_t aidx = node_index(p, ax, ay, az);
       const bool awall = p.flags[aidx] != 0;
       C* cap = (C*)p.oflow + (size_t)e.U[0] * 5 * (size_t)p.oflow_n + t;
       int j = 0;
@@ -1509,6 +1567,16 @@ __device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, c
   p.per[2] = d.per[2];
   p.N = (uint32_t)d.nx * (uint32_t)d.ny * (uint32_t)d.nz;
   p.stride = d.stride;
+  // batch solves keep the default layout, x fastest
+  p.lay.st[0] = 1u;
+  p.lay.st[1] = (uint32_t)d.nx;
+  p.lay.st[2] = (uint32_t)d.nx * (uint32_t)d.ny;
+  p.lay.ord[0] = 0;
+  p.lay.ord[1] = 1;
+  p.lay.ord[2] = 2;
+  p.ln[0] = d.nx;
+  p.ln[1] = d.ny;
+  p.ln[2] = d.nz;
   p.fin = (const void*)((const S*)bp.arena[parity] + d.f_off);
   p.fout = (void*)((S*)bp.arena[parity ^ 1] + d.f_off);
   p.flags = bp.flags + d.flag_off;
@@ -1570,7 +1638,7 @@ __device__ __forceinline__ void batch_node(const BatchParams& bp, const BatchDom
   C f[Q];
   gather_select<S, MODE_FUSED>(p, bo, a, wbits, f);
   collide_node<S, COLL, MODE_FUSED>(p, n, wall, f);
-  store_bulk<S, MODE_FUSED>(p, bo, a, wbits, f);
+  store_bulk<S, MODE_FUSED>(p, bo, n, a, wbits, f);  // batch solves: default layout
 }
 
 // One launch per collision model and time step: the BGK launch walks the

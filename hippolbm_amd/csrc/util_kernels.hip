@@ -33,12 +33,13 @@ __device__ __forceinline__ bool obstacle_solid(const ObstacleDev& o, double px, 
   return o.p0[0] < px && px < o.p1[0] && o.p0[1] < py && py < o.p1[1] && o.p0[2] < pz && pz < o.p1[2];
 }
 
-__global__ void k_rasterize(uint8_t* flags, int nx, int ny, int nz, double dx, const ObstacleDev* obs, int nobs) {
+__global__ void k_rasterize(uint8_t* flags, int nx, int ny, int nz, double dx, const ObstacleDev* obs, int nobs,
+                            NodeLayout L) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
   const int z = blockIdx.z;
   if (x >= nx || y >= ny) return;
-  const size_t idx = ((size_t)z * ny + y) * nx + x;
+  const size_t idx = node_at(L, x, y, z);
   // local index i = node + 2, offset = -2: (double(i) + double(-2)) * dx
   const double px = (double(x + 2) + double(-2)) * dx;
   const double py = (double(y + 2) + double(-2)) * dx;
@@ -92,7 +93,8 @@ __global__ void k_rasterize_ghost(uint8_t* gflags, int nx, int ny, int nz, int p
   if (solid) gflags[ghost_index(x, y, z, nx, ny)] = 1;
 }
 
-__global__ void k_ghost_images(uint8_t* gflags, const uint8_t* flags, int nx, int ny, int nz, int px, int py, int pz) {
+__global__ void k_ghost_images(uint8_t* gflags, const uint8_t* flags, int nx, int ny, int nz, int px, int py, int pz,
+                               NodeLayout L) {
   int x, y, z;
   if (!ghost_node(blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y, blockIdx.z, nx, ny,
                   nz, px, py, pz, x, y, z))
@@ -100,7 +102,7 @@ __global__ void k_ghost_images(uint8_t* gflags, const uint8_t* flags, int nx, in
   const int ix = x < 0 ? nx - 1 : (x >= nx ? 0 : x);
   const int iy = y < 0 ? ny - 1 : (y >= ny ? 0 : y);
   const int iz = z < 0 ? nz - 1 : (z >= nz ? 0 : z);
-  gflags[ghost_index(x, y, z, nx, ny)] = flags[((size_t)iz * ny + iy) * nx + ix];
+  gflags[ghost_index(x, y, z, nx, ny)] = flags[node_at(L, ix, iy, iz)];
 }
 
 // ---------------------------------------------------------------------------
@@ -124,7 +126,7 @@ __global__ void k_ghost_images(uint8_t* gflags, const uint8_t* flags, int nx, in
 // ---------------------------------------------------------------------------
 __global__ void k_find_special(const uint8_t* flags, const uint8_t* gflags, int nx, int ny, int nz, int px, int py,
                                int pz, uint32_t* bbm, uint32_t* out_idx, uint32_t* out_mask, uint32_t* count,
-                               uint32_t cap) {
+                               uint32_t cap, NodeLayout L) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
   const int z = blockIdx.z;
@@ -132,7 +134,7 @@ __global__ void k_find_special(const uint8_t* flags, const uint8_t* gflags, int 
   const bool face = (px && (x == 0 || x == nx - 1)) || (py && (y == 0 || y == ny - 1)) ||
                     (pz && (z == 0 || z == nz - 1));
   if (!face) return;
-  const size_t idx = ((size_t)z * ny + y) * nx + x;
+  const size_t idx = node_at(L, x, y, z);
   if (flags[idx]) return;
   uint32_t m = 0;
   static_for<1, Q>([&](auto ic) {
@@ -158,7 +160,7 @@ __global__ void k_find_special(const uint8_t* flags, const uint8_t* gflags, int 
       sz = sz < 0 ? nz - 1 : 0;
       crossed = true;
     }
-    if (!crossed || !flags[((size_t)sz * ny + sy) * nx + sx]) return;
+    if (!crossed || !flags[node_at(L, sx, sy, sz)]) return;
     m |= gflags[ghost_index(gx, gy, gz, nx, ny)] ? (1u << 31) : (1u << i);
   });
   if (!m) return;
@@ -172,13 +174,13 @@ __global__ void k_find_special(const uint8_t* flags, const uint8_t* gflags, int 
 
 // set_distribution (set_distribution.hpp:38-41): f_i = value * w_i on a node box.
 template <class S>
-__global__ void k_set_dist(void* fv, uint64_t stride, int nx, int ny, int lo0, int lo1, int lo2, int hi0, int hi1,
+__global__ void k_set_dist(void* fv, uint64_t stride, NodeLayout L, int lo0, int lo1, int lo2, int hi0, int hi1,
                            int hi2, double value, RestShift sh) {
   const int x = lo0 + blockIdx.x * blockDim.x + threadIdx.x;
   const int y = lo1 + blockIdx.y * blockDim.y + threadIdx.y;
   const int z = lo2 + blockIdx.z;
   if (x > hi0 || y > hi1 || z > hi2) return;
-  const size_t idx = ((size_t)z * ny + y) * nx + x;
+  const size_t idx = node_at(L, x, y, z);
   S* f = (S*)fv;
 #pragma unroll
   for (int i = 0; i < Q; i++) {
@@ -195,12 +197,12 @@ __global__ void k_set_dist(void* fv, uint64_t stride, int nx, int ny, int lo0, i
 // in the reference's evaluation order.
 template <class S>
 __global__ void k_double_couette(void* fv, uint64_t stride, int nx, int ny, int nz, int dim, double dUx, double dUy,
-                                 double dUz, double Ux, double Uy, double Uz, RestShift sh) {
+                                 double dUz, double Ux, double Uy, double Uz, RestShift sh, NodeLayout L) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
   const int z = blockIdx.z;
   if (x >= nx || y >= ny || z >= nz) return;
-  const size_t idx = ((size_t)z * ny + y) * nx + x;
+  const size_t idx = node_at(L, x, y, z);
   const double value = dim == 0 ? x : (dim == 1 ? y : z);
   const double ux = Ux - dUx * value, uy = Uy - dUy * value, uz = Uz - dUz * value;
   const double u_squ = ux * ux + uy * uy + uz * uz;
@@ -218,24 +220,24 @@ __global__ void k_double_couette(void* fv, uint64_t stride, int nx, int ny, int 
 
 template <class S>
 static hipError_t double_couette_t(void* f, uint64_t stride, int nx, int ny, int nz, int dim, const double* dU,
-                                   const double* U, RestShift sh, hipStream_t s) {
+                                   const double* U, RestShift sh, NodeLayout L, hipStream_t s) {
   dim3 b(64, 4, 1), g((nx + 63) / 64, (ny + 3) / 4, nz);
   hipLaunchKernelGGL((k_double_couette<S>), g, b, 0, s, f, stride, nx, ny, nz, dim, dU[0], dU[1], dU[2], U[0], U[1],
-                     U[2], sh);
+                     U[2], sh, L);
   return hipGetLastError();
 }
 
 hipError_t launch_double_couette(int storage, void* f, uint64_t stride, int nx, int ny, int nz, int dim,
-                                 const double* dU, const double* U, RestShift sh, hipStream_t s) {
+                                 const double* dU, const double* U, RestShift sh, NodeLayout L, hipStream_t s) {
   switch (storage) {
     case 0:
-      return double_couette_t<double>(f, stride, nx, ny, nz, dim, dU, U, sh, s);
+      return double_couette_t<double>(f, stride, nx, ny, nz, dim, dU, U, sh, L, s);
     case 1:
-      return double_couette_t<float>(f, stride, nx, ny, nz, dim, dU, U, sh, s);
+      return double_couette_t<float>(f, stride, nx, ny, nz, dim, dU, U, sh, L, s);
     case 2:
-      return double_couette_t<_Float16>(f, stride, nx, ny, nz, dim, dU, U, sh, s);
+      return double_couette_t<_Float16>(f, stride, nx, ny, nz, dim, dU, U, sh, L, s);
     case 3:
-      return double_couette_t<bf16s>(f, stride, nx, ny, nz, dim, dU, U, sh, s);
+      return double_couette_t<bf16s>(f, stride, nx, ny, nz, dim, dU, U, sh, L, s);
   }
   return hipErrorInvalidValue;
 }
@@ -384,20 +386,19 @@ __global__ void k_reduce_final(const double* part, int nparts, double* out) {
 // plane folds its columns in a fixed order (deterministic sums).
 __global__ void __launch_bounds__(256)
     k_profile_cols(const double* m1, const uint8_t* flags, uint32_t N, int nx, int ny, int nz, int dim,
-                   double scale, ProfileStat* part) {
+                   double scale, ProfileStat* part, NodeLayout L) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int a = blockIdx.y;
   if (x >= nx) return;
-  const size_t nxy = (size_t)nx * ny;
   size_t idx, step;
   int nb;
   if (dim == 1) {  // a = y, b = z
-    idx = (size_t)a * nx + x;
-    step = nxy;
+    idx = node_at(L, x, a, 0);
+    step = L.st[2];
     nb = nz;
   } else {  // a = z, b = y
-    idx = (size_t)a * nxy + x;
-    step = nx;
+    idx = node_at(L, x, 0, a);
+    step = L.st[1];
     nb = ny;
   }
   double sum = 0.0, vmin = 1.7976931348623157e308, vmax = -1.7976931348623157e308;
@@ -469,13 +470,13 @@ __global__ void __launch_bounds__(256)
 // plot_line_velocity (src/check/plot_over_line.cu:77-90): scale*u at the
 // nodes start + k*e_axis; nodes outside the domain give 0.
 __global__ void k_line_velocity(const double* m1, uint32_t N, int nx, int ny, int nz, int x0, int y0, int z0,
-                                int axis, int n, double scale, double* out) {
+                                int axis, int n, double scale, double* out, NodeLayout L) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int x = x0 + (axis == 0 ? k : 0), y = y0 + (axis == 1 ? k : 0), z = z0 + (axis == 2 ? k : 0);
   double u[3] = {0.0, 0.0, 0.0};
   if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz) {
-    const size_t idx = ((size_t)z * ny + y) * nx + x;
+    const size_t idx = node_at(L, x, y, z);
     for (int c = 0; c < 3; c++) u[c] = scale * m1[(size_t)c * N + idx];
   }
   for (int c = 0; c < 3; c++) out[3 * (size_t)k + c] = u[c];
@@ -496,12 +497,13 @@ __global__ void __launch_bounds__(256) hbm_copy_kernel(uint4* __restrict__ dst, 
 // node x - c_i (periodic wrap applied) is a wall.  An upstream node outside a
 // non-periodic face sets no bit: the face bounce-back covers it.
 // ---------------------------------------------------------------------------
-__global__ void k_build_bbm(const uint8_t* flags, int nx, int ny, int nz, int px, int py, int pz, uint32_t* bbm) {
+__global__ void k_build_bbm(const uint8_t* flags, int nx, int ny, int nz, int px, int py, int pz, uint32_t* bbm,
+                            NodeLayout L) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int y = blockIdx.y * blockDim.y + threadIdx.y;
   const int z = blockIdx.z;
   if (x >= nx || y >= ny) return;
-  const size_t idx = ((size_t)z * ny + y) * nx + x;
+  const size_t idx = node_at(L, x, y, z);
   uint32_t m = 0;
   if (flags[idx]) {
     m = 1u;
@@ -521,7 +523,7 @@ __global__ void k_build_bbm(const uint8_t* flags, int nx, int ny, int nz, int px
         if (!pz) return;
         sz = sz < 0 ? nz - 1 : 0;
       }
-      if (flags[((size_t)sz * ny + sy) * nx + sx]) m |= 1u << i;
+      if (flags[node_at(L, sx, sy, sz)]) m |= 1u << i;
     });
   }
   bbm[idx] = m;
@@ -531,16 +533,16 @@ __global__ void k_build_bbm(const uint8_t* flags, int nx, int ny, int nz, int px
 // host launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_build_bbm(const uint8_t* flags, int nx, int ny, int nz, const int* per, uint32_t* bbm,
-                            hipStream_t s) {
+                            NodeLayout L, hipStream_t s) {
   dim3 b(64, 4, 1), g((nx + 63) / 64, (ny + 3) / 4, nz);
-  hipLaunchKernelGGL(k_build_bbm, g, b, 0, s, flags, nx, ny, nz, per[0], per[1], per[2], bbm);
+  hipLaunchKernelGGL(k_build_bbm, g, b, 0, s, flags, nx, ny, nz, per[0], per[1], per[2], bbm, L);
   return hipGetLastError();
 }
 
 hipError_t launch_rasterize(uint8_t* flags, int nx, int ny, int nz, double dx, const ObstacleDev* obs, int nobs,
-                            hipStream_t s) {
+                            NodeLayout L, hipStream_t s) {
   dim3 b(64, 4, 1), g((nx + 63) / 64, (ny + 3) / 4, nz);
-  hipLaunchKernelGGL(k_rasterize, g, b, 0, s, flags, nx, ny, nz, dx, obs, nobs);
+  hipLaunchKernelGGL(k_rasterize, g, b, 0, s, flags, nx, ny, nz, dx, obs, nobs, L);
   return hipGetLastError();
 }
 
@@ -554,42 +556,42 @@ hipError_t launch_rasterize_ghost(uint8_t* gflags, int nx, int ny, int nz, const
 }
 
 hipError_t launch_ghost_images(uint8_t* gflags, const uint8_t* flags, int nx, int ny, int nz, const int* per,
-                               hipStream_t s) {
+                               NodeLayout L, hipStream_t s) {
   hipLaunchKernelGGL(k_ghost_images, padded_grid(nx, ny, nz), dim3(64, 4, 1), 0, s, gflags, flags, nx, ny, nz, per[0],
-                     per[1], per[2]);
+                     per[1], per[2], L);
   return hipGetLastError();
 }
 
 hipError_t launch_find_special(const uint8_t* flags, const uint8_t* gflags, int nx, int ny, int nz, const int* per,
                                uint32_t* bbm, uint32_t* out_idx, uint32_t* out_mask, uint32_t* count, uint32_t cap,
-                               hipStream_t s) {
+                               NodeLayout L, hipStream_t s) {
   dim3 b(64, 4, 1), g((nx + 63) / 64, (ny + 3) / 4, nz);
   hipLaunchKernelGGL(k_find_special, g, b, 0, s, flags, gflags, nx, ny, nz, per[0], per[1], per[2], bbm, out_idx,
-                     out_mask, count, cap);
+                     out_mask, count, cap, L);
   return hipGetLastError();
 }
 
 template <class S>
-static hipError_t set_dist_t(void* f, uint64_t stride, int nx, int ny, const int* lo, const int* hi, double value,
+static hipError_t set_dist_t(void* f, uint64_t stride, NodeLayout L, const int* lo, const int* hi, double value,
                              RestShift sh, hipStream_t s) {
   const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
   dim3 b(64, 4, 1), g((ex + 63) / 64, (ey + 3) / 4, ez);
-  hipLaunchKernelGGL((k_set_dist<S>), g, b, 0, s, f, stride, nx, ny, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2],
+  hipLaunchKernelGGL((k_set_dist<S>), g, b, 0, s, f, stride, L, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2],
                      value, sh);
   return hipGetLastError();
 }
 
-hipError_t launch_set_distribution(int storage, void* f, uint64_t stride, int nx, int ny, const int* lo,
+hipError_t launch_set_distribution(int storage, void* f, uint64_t stride, NodeLayout L, const int* lo,
                                    const int* hi, double value, RestShift sh, hipStream_t s) {
   switch (storage) {
     case 0:
-      return set_dist_t<double>(f, stride, nx, ny, lo, hi, value, sh, s);
+      return set_dist_t<double>(f, stride, L, lo, hi, value, sh, s);
     case 1:
-      return set_dist_t<float>(f, stride, nx, ny, lo, hi, value, sh, s);
+      return set_dist_t<float>(f, stride, L, lo, hi, value, sh, s);
     case 2:
-      return set_dist_t<_Float16>(f, stride, nx, ny, lo, hi, value, sh, s);
+      return set_dist_t<_Float16>(f, stride, L, lo, hi, value, sh, s);
     case 3:
-      return set_dist_t<bf16s>(f, stride, nx, ny, lo, hi, value, sh, s);
+      return set_dist_t<bf16s>(f, stride, L, lo, hi, value, sh, s);
   }
   return hipErrorInvalidValue;
 }
@@ -618,6 +620,45 @@ hipError_t launch_convert(int storage, bool to_store, void* fplane, double* tmp,
       return conv_t<bf16s>(to_store, fplane, tmp, n, q, sig, s);
   }
   return hipErrorInvalidValue;
+}
+
+// One per-node array (esize bytes per node) from layout `from` into layout
+// `to`; threads walk the source order (coalesced reads).
+template <class T>
+__global__ void k_permute(T* dst, const T* src, uint32_t N, int n0, int n1, NodeLayout from, NodeLayout to) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const uint32_t n01 = (uint32_t)n0 * (uint32_t)n1;
+  const uint32_t c2 = i / n01, r = i - c2 * n01, c1 = r / (uint32_t)n0, c0 = r - c1 * (uint32_t)n0;
+  int c[3];
+  c[0] = from.ord[0] == 0 ? c0 : (from.ord[1] == 0 ? c1 : c2);
+  c[1] = from.ord[0] == 1 ? c0 : (from.ord[1] == 1 ? c1 : c2);
+  c[2] = from.ord[0] == 2 ? c0 : (from.ord[1] == 2 ? c1 : c2);
+  dst[node_at(to, c[0], c[1], c[2])] = src[i];
+}
+
+hipError_t launch_permute(void* dst, const void* src, size_t esize, const int n[3], NodeLayout from, NodeLayout to,
+                          hipStream_t s) {
+  const uint32_t N = (uint32_t)n[0] * (uint32_t)n[1] * (uint32_t)n[2];
+  const int n0 = n[from.ord[0]], n1 = n[from.ord[1]];
+  dim3 b(256), g((N + 255) / 256);
+  switch (esize) {
+    case 1:
+      hipLaunchKernelGGL((k_permute<uint8_t>), g, b, 0, s, (uint8_t*)dst, (const uint8_t*)src, N, n0, n1, from, to);
+      break;
+    case 2:
+      hipLaunchKernelGGL((k_permute<uint16_t>), g, b, 0, s, (uint16_t*)dst, (const uint16_t*)src, N, n0, n1, from, to);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_permute<uint32_t>), g, b, 0, s, (uint32_t*)dst, (const uint32_t*)src, N, n0, n1, from, to);
+      break;
+    case 8:
+      hipLaunchKernelGGL((k_permute<uint64_t>), g, b, 0, s, (uint64_t*)dst, (const uint64_t*)src, N, n0, n1, from, to);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 // Shifted storage moving to another rest state: g += delta[class(i)].
@@ -689,10 +730,10 @@ int reduce_parts(uint32_t N) {
 size_t profile_parts(int nx, int ny, int nz, int dim) { return (size_t)nx * (dim == 1 ? ny : nz); }
 
 hipError_t launch_plane_profile(const double* m1, const uint8_t* flags, uint32_t N, int nx, int ny, int nz, int dim,
-                                double scale, ProfileStat* part, ProfileStat* out, hipStream_t s) {
+                                double scale, ProfileStat* part, ProfileStat* out, NodeLayout L, hipStream_t s) {
   const int na = dim == 1 ? ny : nz;
   hipLaunchKernelGGL(k_profile_cols, dim3((unsigned)((nx + 255) / 256), (unsigned)na), dim3(256), 0, s, m1, flags, N,
-                     nx, ny, nz, dim, scale, part);
+                     nx, ny, nz, dim, scale, part, L);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const int planes = dim == 0 ? nx : na;
@@ -731,9 +772,9 @@ hipError_t profile_scratch(ProfileStat** out) {
 }
 
 hipError_t launch_line_velocity(const double* m1, uint32_t N, int nx, int ny, int nz, const int* start, int axis,
-                                int n, double scale, double* out, hipStream_t s) {
+                                int n, double scale, double* out, NodeLayout L, hipStream_t s) {
   hipLaunchKernelGGL(k_line_velocity, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m1, N, nx, ny, nz, start[0],
-                     start[1], start[2], axis, n, scale, out);
+                     start[1], start[2], axis, n, scale, out, L);
   return hipGetLastError();
 }
 

@@ -68,10 +68,11 @@ struct ProfileStat {
   int32_t count, pad;
 };
 
+// Kernels over the node grid take the domain's NodeLayout (d3q19.hpp).
 hipError_t launch_rasterize(uint8_t* flags, int nx, int ny, int nz, double dx, const ObstacleDev* obs, int nobs,
-                            hipStream_t s);
+                            NodeLayout L, hipStream_t s);
 hipError_t launch_build_bbm(const uint8_t* flags, int nx, int ny, int nz, const int* per, uint32_t* bbm,
-                            hipStream_t s);
+                            NodeLayout L, hipStream_t s);
 // Ghost flags: a byte per node of the grid padded by one layer on every axis,
 // index ((z+1)*(ny+2) + (y+1))*(nx+2) + (x+1); only the entries outside the
 // grid on periodic axes are used.  rasterize_ghost writes the obstacles' walls
@@ -81,19 +82,22 @@ inline size_t ghost_count(int nx, int ny, int nz) { return (size_t)(nx + 2) * (n
 hipError_t launch_rasterize_ghost(uint8_t* gflags, int nx, int ny, int nz, const int* per, double dx,
                                   const ObstacleDev* obs, int nobs, hipStream_t s);
 hipError_t launch_ghost_images(uint8_t* gflags, const uint8_t* flags, int nx, int ny, int nz, const int* per,
-                               hipStream_t s);
+                               NodeLayout L, hipStream_t s);
 // Nodes next to a wall across a periodic face (see find_special in
 // util_kernels.hip); appends (node, direction mask) pairs through *count.
 hipError_t launch_find_special(const uint8_t* flags, const uint8_t* gflags, int nx, int ny, int nz, const int* per,
                                uint32_t* bbm, uint32_t* out_idx, uint32_t* out_mask, uint32_t* count, uint32_t cap,
-                               hipStream_t s);
+                               NodeLayout L, hipStream_t s);
 // The initial-state and conversion kernels write and read shifted storage
 // relative to the rest state sh (RestShift, d3q19.hpp); FP64 storage ignores it.
-hipError_t launch_set_distribution(int storage, void* f, uint64_t stride, int nx, int ny, const int* lo,
+hipError_t launch_set_distribution(int storage, void* f, uint64_t stride, NodeLayout L, const int* lo,
                                    const int* hi, double value, RestShift sh, hipStream_t s);
 hipError_t launch_double_couette(int storage, void* f, uint64_t stride, int nx, int ny, int nz, int dim,
-                                 const double* dU, const double* U, RestShift sh, hipStream_t s);
+                                 const double* dU, const double* U, RestShift sh, NodeLayout L, hipStream_t s);
 hipError_t launch_convert(int storage, bool to_store, void* fplane, double* tmp, uint32_t n, int q, RestShift sh,
+                          hipStream_t s);
+// a per-node array of esize (1, 2, 4, 8) bytes from layout `from` into `to`
+hipError_t launch_permute(void* dst, const void* src, size_t esize, const int n[3], NodeLayout from, NodeLayout to,
                           hipStream_t s);
 // shifted storage: g_i += delta[class(i)] over N nodes (a change of rest state)
 hipError_t launch_reshift(int storage, void* f, uint64_t stride, uint32_t N, RestShift delta, hipStream_t s);
@@ -107,7 +111,7 @@ hipError_t launch_reduce(const double* m0, const double* m1, const uint8_t* flag
 // column partials needed by launch_plane_profile (ProfileStat entries)
 size_t profile_parts(int nx, int ny, int nz, int dim);
 hipError_t launch_plane_profile(const double* m1, const uint8_t* flags, uint32_t N, int nx, int ny, int nz, int dim,
-                                double scale, ProfileStat* part, ProfileStat* out, hipStream_t s);
+                                double scale, ProfileStat* part, ProfileStat* out, NodeLayout L, hipStream_t s);
 // The library-owned column partials of hlbm_plane_profile_noalloc on the
 // current device (HLBM_PROFILE_SCRATCH_COLUMNS records); no allocation.
 hipError_t profile_scratch(ProfileStat** out);
@@ -116,6 +120,6 @@ static_assert(offsetof(ProfileStat, vmax) == offsetof(hlbm_plane_stat, vmax) &&
                   offsetof(ProfileStat, count) == offsetof(hlbm_plane_stat, count),
               "hlbm_plane_stat is ProfileStat's ABI form");
 hipError_t launch_line_velocity(const double* m1, uint32_t N, int nx, int ny, int nz, const int* start, int axis,
-                                int n, double scale, double* out, hipStream_t s);
+                                int n, double scale, double* out, NodeLayout L, hipStream_t s);
 
 }  // namespace hlbm

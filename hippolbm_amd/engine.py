@@ -108,6 +108,16 @@ class Domain:
         check("hlbm_set_collision", lib().hlbm_set_collision(self.h, _lib.COLLISION[collision],
                                                              _dp(r) if r is not None else None))
 
+    def set_layout(self, order):
+        """Node layout in HBM: the axes (0 x, 1 y, 2 z) from fastest to slowest."""
+        o = np.ascontiguousarray(order, dtype=np.int32)
+        check("hlbm_set_layout", lib().hlbm_set_layout(self.h, o.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+
+    def get_layout(self):
+        o = np.zeros(3, dtype=np.int32)
+        check("hlbm_get_layout", lib().hlbm_get_layout(self.h, o.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        return tuple(int(v) for v in o)
+
     def set_params(self, tau, fext):
         F = np.ascontiguousarray(fext, dtype=np.float64)
         check("hlbm_set_params", lib().hlbm_set_params(self.h, float(tau), _dp(F)))
@@ -179,14 +189,15 @@ class Domain:
         return t.value
 
     def autotune(self, reps=5, stream=None):
-        """Pick the fastest launch shape for this domain (cached per shape);
-        returns the chosen shape and the per-step times."""
+        """Pick the fastest node layout and launch shape for this domain (cached
+        per shape); returns the choice and the per-step times."""
         t = _lib.TuneInfo()
         s = ctypes.c_void_p(stream) if stream else None
         check("hlbm_autotune", lib().hlbm_autotune(self.h, int(reps), s, ctypes.byref(t)))
+        order = (t.order // 100, t.order // 10 % 10, t.order % 10)
         return {"block": (t.block_x, t.block_y), "npt": t.npt, "xcd_remap": t.xcd_remap, "variant": t.variant,
-                "candidates": t.candidates, "relocated": t.relocated, "ms_best": t.ms_best,
-                "ms_default": t.ms_default}
+                "order": "".join("xyz"[a] for a in order), "candidates": t.candidates, "relocated": t.relocated,
+                "ms_best": t.ms_best, "ms_default": t.ms_default}
 
     def set_launch(self, block=(128, 2), npt=1, xcd_remap=1, variant=0):
         """Launch shape by hand; variant 0 = scalar plane bases + 32-bit offsets,

@@ -338,7 +338,10 @@ int hlbm_batch_info_get(hlbm_batch* b, hlbm_batch_info* out);
 double hlbm_clock(void);
 
 /* ---- launch-shape autotuner (no counterpart in the reference) -----------
- * The step kernels take a tile shape (block_x nodes along x by block_y rows),
+ * A domain with boundary-operator planes first tries the node layouts that
+ * put each such plane's axis last (hlbm_set_layout), timed with the default
+ * shape, and keeps the fastest.  Then, in that layout:
+ * the step kernels take a tile shape (block_x nodes along x by block_y rows),
  * a count of nodes per thread along y (npt, 1 or 2), an XCD-aware tile order
  * and a bulk-kernel variant: 0 = plane bases in scalar registers with 32-bit
  * lane offsets and a per-node bounce-back mask (fewest instructions), 1 =
@@ -364,7 +367,7 @@ typedef struct {
   int32_t candidates;                       /* shapes timed; 0 = taken from the cache */
   int32_t variant;                          /* bulk-kernel variant, 0 or 1 */
   int32_t relocated;                        /* population buffers moved by the placement check */
-  int32_t pad;
+  int32_t order;                            /* chosen node layout, 100*order[0] + 10*order[1] + order[2] */
   double ms_best;                           /* per step, chosen shape */
   double ms_default;                        /* per step, the default 128x2 / npt 1 / remap */
 } hlbm_tune_info;
@@ -375,6 +378,19 @@ int hlbm_autotune(hlbm_domain* h, int32_t reps, void* stream, hlbm_tune_info* ou
  * start_iteration boxes, always take variant 1). */
 int hlbm_set_launch(hlbm_domain* h, int32_t block_x, int32_t block_y, int32_t npt, int32_t xcd_remap,
                     int32_t variant);
+
+/* Node layout in HBM (no counterpart in the reference, whose fields are x
+ * fastest, box3d.hpp:127-132): order = the axes from fastest to slowest in
+ * memory, a permutation of {0 (x), 1 (y), 2 (z)}; {0, 1, 2} at creation.  The
+ * bulk kernel's lanes run along order[0].  Putting the axis of x-normal (or
+ * y-normal) boundary planes last makes those planes contiguous, so the
+ * boundary-operator kernel reads and writes whole cache lines.  The state,
+ * flags and rho/u fields move to the new layout (a pending streaming is
+ * applied first); every host-side array of this ABI stays in node layout (x
+ * fastest), and results do not depend on the layout.  hlbm_autotune tries the
+ * layouts that put a boundary-plane axis last. */
+int hlbm_set_layout(hlbm_domain* h, const int32_t order[3]);
+int hlbm_get_layout(const hlbm_domain* h, int32_t order[3]);
 
 /* ---- measurement helper (no counterpart in the reference) --------------
  * Times `reps` streaming copies of `bytes` bytes between two scratch buffers
