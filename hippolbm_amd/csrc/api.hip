@@ -81,6 +81,10 @@ struct hlbm_domain {
   int force_wide = 0;
   int variant = 0;  // bulk-kernel variant (hlbm_set_launch)
   int boundary_first = 1;  // program-plane kernel before the bulk kernel (measured 1-2 % faster)
+  // the program-plane kernel on a second stream beside the bulk kernel
+  int overlap = 1;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int device = 0;
   uint64_t bytes = 0;
   NodeLayout lay{};  // node layout in HBM (hlbm_set_layout; default x fastest)
@@ -298,6 +302,47 @@ static int ensure_bbm(hlbm_domain* h, hipStream_t s) {
   return check_special(h);
 }
 
+// One step's kernels.  With program planes and a bulk kernel, the program-plane
+// kernel runs on the domain's side stream, forked from and joined back into s:
+// it is latency-bound on few nodes, and the two kernels touch disjoint nodes
+// (A/B) or disjoint slots (AA).  The AA outflow capture runs first, on s.
+static hipError_t run_kernels(hlbm_domain* h, StepParams& p, int mode, hipStream_t s) {
+  const int coll = h->d.collision == HLBM_MRT ? COLL_MRT : COLL_BGK;
+  auto go = [&](hipStream_t st) {
+    return h->d.storage == HLBM_F64 ? launch_step_f64(coll, mode, h->npt, grid_for(h), h->block, p, st)
+                                    : launch_step_lowp(h->d.storage, coll, mode, h->npt, grid_for(h), h->block, p, st);
+  };
+  const bool split = h->overlap && p.prog_planes && mode != MODE_COLLIDE && mode != MODE_AA_FIX;
+  if (split && !h->side) {
+    if (hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming) != hipSuccess)
+      h->overlap = 0;  // keep one stream
+  }
+  if (!split || !h->side) {
+    p.parts = PART_ALL;
+    return go(s);
+  }
+  hipError_t e = hipSuccess;
+  if (p.oflow) {
+    p.parts = PART_CAPTURE;
+    e = go(s);
+  }
+  if (e == hipSuccess) e = hipEventRecord(h->ev_fork, s);
+  if (e == hipSuccess) e = hipStreamWaitEvent(h->side, h->ev_fork, 0);
+  if (e == hipSuccess) {
+    p.parts = PART_BOUNDARY;
+    e = go(h->side);
+  }
+  if (e == hipSuccess) {
+    p.parts = PART_BULK;
+    e = go(s);
+  }
+  if (e == hipSuccess) e = hipEventRecord(h->ev_join, h->side);
+  if (e == hipSuccess) e = hipStreamWaitEvent(s, h->ev_join, 0);
+  return e;
+}
+
 static int launch(hlbm_domain* h, int mode, bool write_macro, hipStream_t s, void* out = nullptr,
                   bool skip_program = false) {
   if (h->any_wall && h->plan.wall_bb) {
@@ -313,12 +358,7 @@ static int launch(hlbm_domain* h, int mode, bool write_macro, hipStream_t s, voi
   const bool in_place = mode == MODE_AA_EVEN || mode == MODE_AA_ODD;
   p.oflow = in_place ? h->oflow : nullptr;
   p.oflow_n = h->oflow_n;
-  hipError_t e;
-  const int coll = h->d.collision == HLBM_MRT ? COLL_MRT : COLL_BGK;
-  if (h->d.storage == HLBM_F64)
-    e = launch_step_f64(coll, mode, h->npt, grid_for(h), h->block, p, s);
-  else
-    e = launch_step_lowp(h->d.storage, coll, mode, h->npt, grid_for(h), h->block, p, s);
+  const hipError_t e = run_kernels(h, p, mode, s);
   if (e != hipSuccess) return fail(HLBM_ERR_HIP, std::string("step kernel launch: ") + hipGetErrorString(e));
   if (!aa) h->cur ^= 1;
   return HLBM_OK;
@@ -408,6 +448,7 @@ int hlbm_domain_create(const hlbm_domain_desc* desc, hlbm_domain** out) {
   // the 64-bit-address variant otherwise runs only on domains above 2^32 bytes per plane
   if (const char* fw = getenv("HLBM_FORCE_WIDE")) h->force_wide = atoi(fw) ? 1 : 0;
   if (const char* bf = getenv("HLBM_BOUNDARY_FIRST")) h->boundary_first = atoi(bf) ? 1 : 0;
+  if (const char* ov = getenv("HLBM_OVERLAP")) h->overlap = atoi(ov) ? 1 : 0;
   (void)hipGetDevice(&h->device);
   const char* blk = getenv("HLBM_BLOCK");
   if (blk) {
@@ -464,6 +505,9 @@ int hlbm_domain_destroy(hlbm_domain* h) {
                   (void*)h->red_out, (void*)h->tmp, (void*)h->bbm, h->oflow, (void*)h->gflags, (void*)h->spec_idx,
                   (void*)h->spec_mask, (void*)h->spec_count, h->spec_val, h->spec_scratch})
     if (p) (void)hipFree(p);
+  if (h->ev_fork) (void)hipEventDestroy(h->ev_fork);
+  if (h->ev_join) (void)hipEventDestroy(h->ev_join);
+  if (h->side) (void)hipStreamDestroy(h->side);
   delete h;
   return HLBM_OK;
 }
@@ -1178,7 +1222,6 @@ std::map<TuneKey, TuneVal> tune_cache;
 static int time_shape(hlbm_domain* h, void* scratch, int reps, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
                       double* ms) {
   const bool aa = h->d.pattern == HLBM_PATTERN_AA;
-  const int coll = h->d.collision == HLBM_MRT ? COLL_MRT : COLL_BGK;
   StepParams p = base_params(h);
   p.write_macro = 0;
   p.skip_program = 0;
@@ -1194,8 +1237,7 @@ static int time_shape(hlbm_domain* h, void* scratch, int reps, hipStream_t s, hi
       p.fin = h->buf[h->cur];
       p.fout = h->buf[h->cur ^ 1];  // overwritten by the next real step anyway
     }
-    if (h->d.storage == HLBM_F64) return launch_step_f64(coll, mode, h->npt, grid_for(h), h->block, p, s);
-    return launch_step_lowp(h->d.storage, coll, mode, h->npt, grid_for(h), h->block, p, s);
+    return run_kernels(h, p, mode, s);
   };
   hipError_t e = hipSuccess;
   for (int k = 0; k < 2 && e == hipSuccess; k++) e = one(k);  // warm-up

@@ -187,6 +187,7 @@ static StepParams single_params(hlbm_batch* b, const Solve& s, int parity) {
   for (int i = 0; i < Q; i++) p.ef[i] = s.dom.ef[i];
   set_fp32_constants(p);
   for (int k = 0; k < 10; k++) p.mrt[k] = s.dom.mrt[k];
+  p.parts = PART_ALL;
   p.gx = (s.dom.nx + 127) / 128;
   p.gy = (s.dom.ny + 1) / 2;
   p.gz = s.dom.nz;

@@ -55,27 +55,30 @@ template <class S, int COLL, int MODE, int NPT, bool WIDE, bool PERM>
 hipError_t step_go(dim3 grid, dim3 block, const StepParams& p, hipStream_t s) {
   hipError_t e = hipSuccess;
   if constexpr (MODE == MODE_AA_EVEN || MODE == MODE_AA_ODD) {
-    if (p.oflow && !p.skip_program) {  // outflow values first: the step below updates nodes in place
+    // outflow values first: the step below updates nodes in place
+    if ((p.parts & PART_CAPTURE) && p.oflow && !p.skip_program) {
       hipLaunchKernelGGL((outflow_capture_kernel<S, MODE>), dim3((p.oflow_n + 255) / 256), dim3(256), 0, s, p);
       e = hipGetLastError();
       if (e != hipSuccess) return e;
     }
   }
-  const bool planes = MODE != MODE_COLLIDE && p.prog_planes;
+  const bool planes = MODE != MODE_COLLIDE && p.prog_planes && (p.parts & PART_BOUNDARY);
   if (planes && p.boundary_first) {
     hipLaunchKernelGGL((boundary_kernel<S, COLL, MODE>), dim3(boundary_blocks(p)), dim3(BOUNDARY_BLOCK), 0, s, p);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
-  if (MODE != MODE_AA_FIX) {  // AA_FIX only touches program-plane nodes
-    hipLaunchKernelGGL((step_kernel<S, COLL, MODE, NPT, WIDE, PERM>), grid, block, 0, s, p);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
-  if (p.nspec) {  // special nodes on program planes are left out of the program-plane kernel too
-    hipLaunchKernelGGL((special_kernel<S, COLL, MODE>), dim3((p.nspec + 63) / 64), dim3(64), 0, s, p);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
+  if (p.parts & PART_BULK) {
+    if (MODE != MODE_AA_FIX) {  // AA_FIX only touches program-plane nodes
+      hipLaunchKernelGGL((step_kernel<S, COLL, MODE, NPT, WIDE, PERM>), grid, block, 0, s, p);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    if (p.nspec) {  // special nodes on program planes are left out of the program-plane kernel too
+      hipLaunchKernelGGL((special_kernel<S, COLL, MODE>), dim3((p.nspec + 63) / 64), dim3(64), 0, s, p);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
   }
   if (!planes || p.boundary_first) return e;
   hipLaunchKernelGGL((boundary_kernel<S, COLL, MODE>), dim3(boundary_blocks(p)), dim3(BOUNDARY_BLOCK), 0, s, p);

@@ -53,6 +53,10 @@ __host__ __device__ constexpr bool mode_aa(int m) { return m >= MODE_AA_EVEN; }
 // in-place AA steps leave wall nodes alone when wall_bounce_back is active:
 // nothing ever reads a wall's populations then
 __host__ __device__ constexpr bool mode_skips_walls(int m) { return m == MODE_AA_EVEN || m == MODE_AA_ODD; }
+// The kernels of one step: the AA outflow capture, the program-plane kernel,
+// and the bulk kernel with special_kernel.  The host may launch the program
+// planes on a second stream beside the bulk (they touch disjoint nodes).
+enum StepPart { PART_CAPTURE = 1, PART_BOUNDARY = 2, PART_BULK = 4, PART_ALL = 7 };
 enum ProgKind { PK_LID = 1, PK_NEUMANN = 2, PK_REBB = 3, PK_PRESSURE = 4, PK_OUTFLOW = 5 };
 enum CollKind { COLL_BGK = 0, COLL_MRT = 1 };
 
@@ -115,6 +119,7 @@ struct StepParams {
   const uint32_t* spec_mask;   // bit i: f_i takes the node's own f_opp(i) of the previous step
   void* spec_val;              // those values (C type), [i * nspec + k]
   int32_t boundary_first;      // launch the program-plane kernel before the bulk kernel
+  int32_t parts;               // kernels of the step to launch (StepPart bits)
   int32_t nset;                // start_iteration set_distribution boxes
   int32_t set_box[MAX_SET][6]; // node box, inclusive: x0 y0 z0 x1 y1 z1
   double set_val[MAX_SET];     // f_i = value * w_i
