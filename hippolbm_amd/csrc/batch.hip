@@ -133,7 +133,7 @@ struct hlbm_batch {
   double* f_tmp = nullptr;  // one FP64 population plane (keep_populations, initial populations)
   double* d_part = nullptr; // simulation_state partial sums (retirement, one reduction at a time on `stream`)
   uint64_t max_nodes = 0;
-  unsigned blocks = 2048;
+  unsigned grid[2] = {8u, 8u};  // workgroups of the BGK and MRT launches (graph captured with these)
   hipStream_t stream = nullptr;       // chunks, admission, retirement
   hipStream_t copy_stream = nullptr;  // result read-back
   hipGraphExec_t graph = nullptr;
@@ -215,8 +215,28 @@ static hipError_t chunk_launch(hlbm_batch* b, int step) {
   bp.step = step;
   bp.xcd_split = b->xcd_split;
   bp.sr = b->sr;
-  if (b->d.storage == HLBM_F64) return launch_batch_f64(b->blocks, bp, b->stream);
-  return launch_batch_lowp(b->d.storage, b->blocks, bp, b->stream);
+  if (b->d.storage == HLBM_F64) return launch_batch_f64(b->grid, bp, b->stream);
+  return launch_batch_lowp(b->d.storage, b->grid, bp, b->stream);
+}
+
+// Launch sizes follow the work: one workgroup per work unit (a persistent grid
+// of resident workgroups walking the units ran the 16-bit kernel 8-13 % slower),
+// with a quarter of headroom so that a mix that grows or shrinks a little keeps
+// its captured graph; more units than workgroups, or fewer than half, resize.
+// Returns whether a size changed.
+static bool size_grids(hlbm_batch* b, uint32_t nbgk, uint32_t nmrt) {
+  const int es = (int)b->esize;
+  const uint64_t units[2] = {(uint64_t)nbgk * batch_units_per_tile(es, COLL_BGK),
+                             (uint64_t)nmrt * batch_units_per_tile(es, COLL_MRT)};
+  bool changed = false;
+  for (int k = 0; k < 2; k++) {
+    const uint64_t need = std::max<uint64_t>(8, round_up(units[k], 8));
+    if (need > b->grid[k] || 2 * need < b->grid[k]) {
+      b->grid[k] = (unsigned)std::min<uint64_t>(round_up(need + need / 4, 8), 0x7FFFFFF8u);
+      changed = true;
+    }
+  }
+  return changed;
 }
 
 static int build_graph(hlbm_batch* b) {
@@ -263,12 +283,7 @@ int hlbm_batch_create(const hlbm_batch_desc* desc, hlbm_batch** out) {
   b->arena_elems = b->max_nodes * 19;
   b->flag_bytes = b->max_nodes + 4096 * 256;
   b->max_doms = 16384;
-  b->max_tiles = b->max_nodes / 256 + b->max_doms;
-  hipDeviceProp_t prop;
-  int dev = 0;
-  (void)hipGetDevice(&dev);
-  if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-    b->blocks = (unsigned)prop.multiProcessorCount * 8u;
+  b->max_tiles = b->max_nodes / BATCH_TILE_NODES + b->max_doms;
   auto cleanup = [&](const char* what) {
     hlbm_batch_destroy(b);
     return bfail(HLBM_ERR_NOMEM, std::string("batch allocation failed: ") + what);
@@ -390,11 +405,10 @@ int hlbm_batch_submit(hlbm_batch* b, const hlbm_solve_desc* s, int64_t* id) {
   d.inv_tau = 1 / s->tau;
   for (int i = 0; i < Q; i++) d.ef[i] = CX(i) * s->fext[0] + CY(i) * s->fext[1] + CZ(i) * s->fext[2];
   d.has_force = 0;
-  for (int i = 0; i < Q; i++) {
-    d.ef3f[i] = 3.0f * (float)d.ef[i];
+  for (int i = 0; i < Q; i++)
     if (d.ef[i] != 0.0) d.has_force = 1;
-  }
-  d.omega_f = (float)(1.0 / d.tau);
+  for (int k = 0; k < 3; k++) d.f3[k] = 3.0f * (float)s->fext[k];
+  bgk_fp32_constants(d.tau, d.k1, d.kw);
   {
     const RestShift sh = rest_shift(s->collision == HLBM_MRT, s->mrt_custom, s->mrt_rates);
     for (int k = 0; k < 3; k++) d.sig[k] = (float)sh.s[k];
@@ -409,6 +423,7 @@ int hlbm_batch_submit(hlbm_batch* b, const hlbm_solve_desc* s, int64_t* id) {
     for (int j = 0; j < 5; j++) {
       const int q = Faces::in[4 + side][j];
       d.cav[side][j] = 6. * WD(q) * (CX(q) * uxx + CY(q) * uyy + CZ(q) * uzz);
+      d.cavf[side][j] = (float)d.cav[side][j];
     }
   }
   *id = v.id;
@@ -638,7 +653,7 @@ int hlbm_batch_run(hlbm_batch* b, int32_t* n_finished) {
       for (size_t k = 0; k < b->running.size(); k++) {
         const Solve& s = b->solves[b->running[k]];
         if (s.dom.coll != (pass == 0 ? COLL_BGK : COLL_MRT)) continue;
-        for (uint32_t first = 0; first < s.N; first += 256) h_tiles[nt++] = BatchTile{(uint32_t)k, first};
+        for (uint32_t first = 0; first < s.N; first += BATCH_TILE_NODES) h_tiles[nt++] = BatchTile{(uint32_t)k, first};
       }
       if (pass == 0) nbgk = nt;
     }
@@ -655,6 +670,15 @@ int hlbm_batch_run(hlbm_batch* b, int32_t* n_finished) {
           hipMemcpyAsync(b->d_ntiles, h_ntiles, 2 * sizeof(uint32_t), hipMemcpyHostToDevice, st) != hipSuccess)
         return bfail(HLBM_ERR_HIP, "upload tiles");
       b->tiles_dirty = false;
+      if (size_grids(b, nbgk, nt - nbgk) && b->d.use_graph) {
+        // the captured launches carry the old sizes: let the chunk that may still
+        // replay them finish, then capture the chunk again
+        if (hipStreamSynchronize(st) != hipSuccess) return bfail(HLBM_ERR_HIP, "sync");
+        if (b->graph) (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+        const int rc2 = build_graph(b);
+        if (rc2) return rc2;
+      }
     }
     int steps_needed = 0;
     for (int64_t id : b->running) steps_needed = std::max(steps_needed, b->solves[id].dom.active_steps);

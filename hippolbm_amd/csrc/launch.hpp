@@ -1,6 +1,7 @@
 // launch.hpp -- host-side launchers for the step kernels (one translation unit
 // per floating-point contract mode).
 #pragma once
+#include <cmath>
 
 #include <hip/hip_runtime.h>
 
@@ -23,13 +24,47 @@ inline bool wide_offsets(const StepParams& p, size_t esize) {
 }
 
 // Uniform constants of the FP32-compute forms, derived from tau and c_i . F.
+// Constants of the FP32 BGK update g' = k1 g + kw_c (dr + rho t_i): kw_1 is
+// the float nearest 1/(18 tau) for which kw_0 = 6 kw_1 and k1 = 1 - 18 kw_1
+// are floats too (kw_2 = kw_1 / 2 always is).  In exact arithmetic the update
+// then keeps mass and momentum (k1 + kw_0 + 6 kw_1 + 12 kw_2 = k1 + 6 kw_1 +
+// 24 kw_2 = 1), so only the rounding of each operation remains; rounding the
+// four constants independently leaves a systematic momentum change of up to
+// 1e-7 per step, which a force-driven channel amplifies by H^2 / nu.
+inline void bgk_fp32_constants(double tau, float& k1, float kw[3]) {
+  const double target = 1.0 / (18.0 * tau);
+  const float base = (float)target;
+  float up = base, dn = base, best = base;
+  bool found = false;
+  for (int k = 0; k < 4096 && !found; k++) {
+    for (int side = 0; side < 2 && !found; side++) {
+      const double d = side ? (double)dn : (double)up;
+      if ((double)(float)(1.0 - 18.0 * d) == 1.0 - 18.0 * d && (double)(float)(6.0 * d) == 6.0 * d) {
+        best = (float)d;
+        found = true;
+      }
+    }
+    up = std::nextafter(up, 2.0f * base + 1.0f);
+    dn = std::nextafter(dn, 0.0f);
+  }
+  kw[1] = best;  // tau outside any practical range falls back to independent rounding
+  kw[2] = best * 0.5f;
+  kw[0] = (float)(6.0 * (double)best);
+  k1 = (float)(1.0 - 18.0 * (double)best);
+}
+
+// c_1, c_3, c_5 are the unit vectors +x, +y, +z, so ef[1], ef[3], ef[5] are F.
+static_assert(CX(1) == 1 && CY(3) == 1 && CZ(5) == 1, "axis directions");
 inline void set_fp32_constants(StepParams& p) {
   p.has_force = 0;
-  for (int i = 0; i < Q; i++) {
-    p.ef3f[i] = 3.0f * (float)p.ef[i];
+  for (int i = 0; i < Q; i++)
     if (p.ef[i] != 0.0) p.has_force = 1;
-  }
-  p.omega_f = (float)(1.0 / p.tau);
+  p.f3[0] = 3.0f * (float)p.ef[1];
+  p.f3[1] = 3.0f * (float)p.ef[3];
+  p.f3[2] = 3.0f * (float)p.ef[5];
+  bgk_fp32_constants(p.tau, p.k1, p.kw);
+  for (int side = 0; side < 2; side++)
+    for (int j = 0; j < 5; j++) p.cavf[side][j] = (float)p.cav[side][j];
 }
 
 // Program-plane launches use small workgroups: a few planes of nodes are too
@@ -122,8 +157,10 @@ hipError_t launch_step_f64(int coll, int mode, int npt, dim3 grid, dim3 block, c
 hipError_t launch_step_lowp(int storage, int coll, int mode, int npt, dim3 grid, dim3 block, const StepParams& p,
                             hipStream_t s);
 
-// batched domains, one persistent launch (blocks = resident workgroups)
-hipError_t launch_batch_f64(unsigned blocks, const BatchParams& bp, hipStream_t s);
-hipError_t launch_batch_lowp(int storage, unsigned blocks, const BatchParams& bp, hipStream_t s);
+// batched domains, one launch per collision model and step; blocks[0],
+// blocks[1]: workgroups of the BGK and the MRT launch (any number is correct:
+// a workgroup walks units grid-stride)
+hipError_t launch_batch_f64(const unsigned blocks[2], const BatchParams& bp, hipStream_t s);
+hipError_t launch_batch_lowp(int storage, const unsigned blocks[2], const BatchParams& bp, hipStream_t s);
 
 }  // namespace hlbm

@@ -103,10 +103,12 @@ struct StepParams {
   int32_t has_force;    // some c_i . F is non-zero (otherwise the force term adds exactly +0)
   int32_t force_wide;   // take the 64-bit-address bulk variant even when 32-bit offsets would do
   int32_t pair32;       // (stride + N) * element size < 2^32: bounce-back waves gather without branches
-  float omega_f;        // 1/tau for the FP32-compute forms
+  float k1;             // 1 - 1/tau (FP32 BGK: g' = k1 g + kw[class] (dr + rho t))
+  float kw[3];          // w_class / tau for the rest, axis and diagonal classes
   float sig[3];         // shifted storage: rest state w_i + sig[class(i)] (RestShift)
-  float ef3f[Q];        // 3 c_i . F for the FP32-compute forms
+  float f3[3];          // 3 F for the FP32-compute forms (the force term is rho w_i c_i . 3F)
   double cav[2][5];
+  float cavf[2][5];     // cav in FP32 for the FP32-compute forms (uniform registers, no conversions)
   double tau;
   double inv_tau;       // 1 / tau (the MRT default s9 = s13, mrt.hpp:43)
   double F[3];
@@ -183,9 +185,15 @@ struct StoreOps<_Float16> {
     // FP16 keeps 10 of FP32's 23 mantissa bits: the 13 dropped ones are dithered
     // with the top 13 bits of r, i.e. (bits(v) + (r >> 19)) & ~0x1FFF, then
     // converted; gfx950's v_cvt_sr_f16_f32 does exactly this in one instruction
+    st_sr_scaled(p, i, v * SCALE, r);
+  }
+  // the same for a value already multiplied by SCALE (the FP32 BGK collision
+  // can fold the scale into its constants).  The instruction keeps the upper
+  // half of its destination, which is not stored: the scaled value's own
+  // register serves, no zeroed one is needed.
+  static __device__ __forceinline__ void st_sr_scaled(_Float16* p, size_t i, C s, uint32_t r) {
     typedef _Float16 h2 __attribute__((ext_vector_type(2)));
-    const h2 z = {(_Float16)0.0f, (_Float16)0.0f};
-    const h2 h = __builtin_amdgcn_cvt_sr_f16_f32(z, v * SCALE, r, false);
+    const h2 h = __builtin_amdgcn_cvt_sr_f16_f32(__builtin_bit_cast(h2, s), s, r, false);
     __builtin_nontemporal_store(h[0], p + i);
   }
 };
@@ -206,8 +214,7 @@ struct StoreOps<bf16s> {
     // the 16 dropped bits dithered with the top 16 bits of r, (bits(v) + (r >> 16)) >> 16:
     // gfx950's v_cvt_sr_bf16_f32
     typedef short s2 __attribute__((ext_vector_type(2)));
-    const s2 z = {0, 0};
-    const s2 b = __builtin_amdgcn_cvt_sr_bf16_f32(z, v, r, false);
+    const s2 b = __builtin_amdgcn_cvt_sr_bf16_f32(__builtin_bit_cast(s2, v), v, r, false);
     __builtin_nontemporal_store((uint16_t)b[0], (uint16_t*)(p + i));
   }
 };
@@ -271,6 +278,15 @@ __device__ __forceinline__ void macro_f64(const double* f, double& rho, double& 
   }
 }
 
+// Moving-wall coefficient j of the cavity face `side` in the compute type.
+template <class C>
+__device__ __forceinline__ C cav_coef(const StepParams& p, int side, int j) {
+  if constexpr (std::is_same<C, double>::value)
+    return p.cav[side][j];
+  else
+    return p.cavf[side][j];
+}
+
 // c_i . v in the reference's order (ex*vx + ey*vy) + ez*vz with zero terms dropped.
 template <int i, class C>
 __device__ __forceinline__ C cdot(C vx, C vy, C vz) {
@@ -330,20 +346,70 @@ __device__ __forceinline__ void bgk_f64(double* f, double rho, double ux, double
   });
 }
 
-// BGK on shifted populations g = f - w, FP32.
+// BGK on shifted populations g = f - w, FP32:
+//   g' = g + omega (g_eq - g) = k1 g + kw_c (dr + rho t_i),
+//   t_i = eu (3 + 4.5 eu) - 1.5 u^2,  g_eq = w_i (dr + rho t_i),
+// with k1 = 1 - omega and kw_c = omega w_c per weight class; kw_c rho and
+// kw_c dr are formed once per node, so a population costs four fused
+// operations.  SO (a power of two) scales the result exactly: with FP16's
+// storage scale the store needs no multiply, and every form rounds alike.
+// The body-force term rho w_i c_i . 3F is a separate pass under the solve's
+// uniform force flag (inside the update it would cost a multiply-add per
+// population on every node and hold 19 scalar registers).
+template <bool SCALED>
 __device__ __forceinline__ void bgk_shift(float* g, float dr, float ux, float uy, float uz, const StepParams& p) {
+  constexpr float SO = SCALED ? StoreOps<_Float16>::SCALE : 1.0f;
   const float rho = 1.0f + dr;
-  const float omega = p.omega_f;
   const float usq15 = 1.5f * (ux * ux + uy * uy + uz * uz);
-  const bool force = p.has_force;
+  const float k1 = p.k1 * SO;
+  // the classes are contiguous (0 | 1..6 | 7..18): their constants are formed
+  // as each begins, so only one pair is live
+  float kr = 0.0f, kd = 0.0f;
   static_for([&](auto ic) {
     constexpr int i = decltype(ic)::value;
-    const float w = WF(i);
+    if constexpr (i == 0 || i == 1 || i == 7) {
+      const float kw = p.kw[dir_class(i)] * SO;
+      kr = kw * rho;
+      kd = kw * dr;
+    }
     const float eu = cdot<i>(ux, uy, uz);
-    const float geq = w * (dr + rho * (eu * (3.0f + 4.5f * eu) - usq15));
-    g[i] = g[i] + omega * (geq - g[i]);
-    if (force) g[i] += rho * w * p.ef3f[i];
+    const float t = eu * (3.0f + 4.5f * eu) - usq15;
+    g[i] = kr * t + (k1 * g[i] + kd);
   });
+  if (p.has_force) {
+    const float fx = p.f3[0], fy = p.f3[1], fz = p.f3[2];
+    static_for([&](auto ic) {
+      constexpr int i = decltype(ic)::value;
+      g[i] += (rho * (WF(i) * SO)) * cdot<i>(fx, fy, fz);
+    });
+  }
+}
+
+// rho and j = sum c_i g_i of FP32 populations: zero coefficients skipped and
+// +-1 coefficients as add / subtract.  The sums start from -0, which x + (-0)
+// leaves unchanged for every x, so the first term costs nothing (a start from
+// +0, or x * 0 terms, are not folded under IEEE semantics: 30 instructions per
+// node).
+__device__ __forceinline__ void moments_f32(const float* g, float& dr, float& jx, float& jy, float& jz) {
+  dr = jx = jy = jz = -0.0f;
+  static_for([&](auto ic) {
+    constexpr int i = decltype(ic)::value;
+    dr += g[i];
+    if constexpr (CX(i) > 0) jx += g[i];
+    if constexpr (CX(i) < 0) jx -= g[i];
+    if constexpr (CY(i) > 0) jy += g[i];
+    if constexpr (CY(i) < 0) jy -= g[i];
+    if constexpr (CZ(i) > 0) jz += g[i];
+    if constexpr (CZ(i) < 0) jz -= g[i];
+  });
+}
+
+// 1 / x for x near 1 (a lattice density): the hardware reciprocal (1 ulp)
+// and one Newton step, which leaves it within half an ulp plus a few units of
+// 2^-46; three instructions instead of the IEEE division's eleven.
+__device__ __forceinline__ float recip_f32(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  return __builtin_fmaf(__builtin_fmaf(-x, r, 1.0f), r, r);
 }
 
 // ---------------------------------------------------------------------------
@@ -780,7 +846,7 @@ __device__ __forceinline__ typename StoreOps<S>::C pull_one(const StepParams& p,
       C v = O::ld(fin, (size_t)(AA ? I : OPP(I)) * st + idx);
       if constexpr (CZ(I) != 0) {
         constexpr int side = CZ(I) == 1 ? 0 : 1;
-        if (sz < 0 && p.cav_gather && p.cav_on[side]) v += C(p.cav[side][cavity_slot(I)]);
+        if (sz < 0 && p.cav_gather && p.cav_on[side]) v += cav_coef<C>(p, side, cavity_slot(I));
       }
       return v;
     }
@@ -831,7 +897,7 @@ __device__ __forceinline__ void apply_program(const StepParams& p, int x, int y,
                   // into slot qi, which already carries the moving-wall term when the gather
                   // adds it (cav_gather); every other state holds the bare f*_opp(qi)(x)
                   constexpr bool natural = MODE == MODE_AA_EVEN || MODE == MODE_AA_FIX;
-                  if (dim == 2 && p.cav_on[side] && !(natural && p.cav_gather)) v += C(p.cav[side][j]);
+                  if (dim == 2 && p.cav_on[side] && !(natural && p.cav_gather)) v += cav_coef<C>(p, side, j);
                   f[qi] = v;
                 });
               }
@@ -879,7 +945,10 @@ __device__ __forceinline__ void apply_program(const StepParams& p, int x, int y,
 }
 
 // Phase 3: macro_variables (optionally written out) and collision.
-template <class S, int COLL, int MODE>
+// RAW (FP16 BGK in the A/B bulk paths only): the populations leave already
+// multiplied by the storage scale, for store_bulk<..., RAW>; wall lanes are
+// scaled as they are.
+template <class S, int COLL, int MODE, bool RAW = false>
 __device__ __forceinline__ void collide_node(const StepParams& p, uint32_t idx, bool wall,
                                              typename StoreOps<S>::C* f) {
   using O = StoreOps<S>;
@@ -907,16 +976,16 @@ __device__ __forceinline__ void collide_node(const StepParams& p, uint32_t idx, 
           mrt_collide<double>((double*)f, rho, p, false);
       }
     } else {
-      float dr = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+      if constexpr (RAW) {
+        if (wall) {  // not collided: stored as they are, in the scaled form
 #pragma unroll
-      for (int i = 0; i < Q; i++) {
-        dr += f[i];
-        jx += float(CX(i)) * f[i];
-        jy += float(CY(i)) * f[i];
-        jz += float(CZ(i)) * f[i];
+          for (int i = 0; i < Q; i++) f[i] *= StoreOps<S>::SCALE;
+        }
       }
+      float dr, jx, jy, jz;
+      moments_f32((const float*)f, dr, jx, jy, jz);
       const float rho = 1.0f + dr;
-      const float inv = 1.0f / rho;
+      const float inv = recip_f32(rho);
       const float ux = jx * inv, uy = jy * inv, uz = jz * inv;
       if (p.write_macro) {
         if (!wall) {
@@ -932,7 +1001,7 @@ __device__ __forceinline__ void collide_node(const StepParams& p, uint32_t idx, 
       }
       if (!wall) {
         if constexpr (COLL == COLL_BGK) {
-          bgk_shift((float*)f, dr, ux, uy, uz, p);
+          bgk_shift<RAW>((float*)f, dr, ux, uy, uz, p);
         } else {
           mrt_collide<float>((float*)f, rho, p, true);
         }
@@ -974,7 +1043,7 @@ __device__ __forceinline__ void store_node(const StepParams& p, int x, int y, in
         C v = f[i];
         if constexpr (CZ(i) != 0) {
           constexpr int side = CZ(i) == -1 ? 0 : 1;  // face of the incoming direction opp(i)
-          if (tz < 0 && p.cav_gather && p.cav_on[side]) v += C(p.cav[side][cavity_slot(OPP(i))]);
+          if (tz < 0 && p.cav_gather && p.cav_on[side]) v += cav_coef<C>(p, side, cavity_slot(OPP(i)));
         }
         O::st_sr(fout, (size_t)OPP(i) * st + idx, v, R.word(i));
       } else {
@@ -1140,7 +1209,7 @@ __device__ __forceinline__ void gather_select(const StepParams& p, uint32_t bo, 
     C v = O::ld(at_bytes(b, bb ? back : pull), 0);
     if constexpr (CZ(i) != 0) {
       constexpr int side = CZ(i) == 1 ? 0 : 1;
-      if (p.cav_gather && p.cav_on[side]) v += (zf ? C(p.cav[side][cavity_slot(i)]) : C(0));
+      if (p.cav_gather && p.cav_on[side]) v += (zf ? cav_coef<C>(p, side, cavity_slot(i)) : C(0));
     }
     f[i] = v;
   });
@@ -1188,7 +1257,7 @@ __device__ __forceinline__ void gather_bulk(const StepParams& p, uint32_t bo, co
           C v = O::ld(at_bytes(fin + (size_t)(AA ? i : OPP(i)) * st, bo), 0);
           if constexpr (CZ(i) != 0) {
             constexpr int side = CZ(i) == 1 ? 0 : 1;
-            if (zf && p.cav_gather && p.cav_on[side]) v += C(p.cav[side][cavity_slot(i)]);
+            if (zf && p.cav_gather && p.cav_on[side]) v += cav_coef<C>(p, side, cavity_slot(i));
           }
           f[i] = v;
         }
@@ -1198,7 +1267,14 @@ __device__ __forceinline__ void gather_bulk(const StepParams& p, uint32_t bo, co
 }
 
 // Store for a bulk node (same result as store_node).
-template <class S, int MODE>
+// The A/B bulk paths of FP16 BGK fold the storage scale into the collision
+// (collide_node<..., RAW>) and store without a multiply (store_bulk<..., RAW>).
+template <class S, int COLL, int MODE>
+constexpr bool raw_store() {
+  return std::is_same<S, _Float16>::value && COLL == COLL_BGK && mode_collides(MODE) && !mode_aa(MODE);
+}
+
+template <class S, int MODE, bool RAW = false>
 __device__ __forceinline__ void store_bulk(const StepParams& p, uint32_t bo, uint32_t cidx, const NodeNbrs& a,
                                            uint32_t wbits, typename StoreOps<S>::C* f) {
   using O = StoreOps<S>;
@@ -1232,7 +1308,7 @@ __device__ __forceinline__ void store_bulk(const StepParams& p, uint32_t bo, uin
           C v = f[i];
           if constexpr (CZ(i) != 0) {
             constexpr int side = CZ(i) == -1 ? 0 : 1;  // face of the incoming direction opp(i)
-            if (zf && p.cav_gather && p.cav_on[side]) v += C(p.cav[side][cavity_slot(OPP(i))]);
+            if (zf && p.cav_gather && p.cav_on[side]) v += cav_coef<C>(p, side, cavity_slot(OPP(i)));
           }
           O::st_sr(at_bytes(fout + (size_t)OPP(i) * st, bo), 0, v, R.word(i));
         } else {
@@ -1244,7 +1320,10 @@ __device__ __forceinline__ void store_bulk(const StepParams& p, uint32_t bo, uin
     static_for([&](auto jc) {
       constexpr int j = decltype(jc)::value;
       b = scalar_ptr(j ? b + st : b);
-      O::st_sr(at_bytes(b, bo), 0, f[j], R.word(j));
+      if constexpr (RAW)
+        O::st_sr_scaled(at_bytes(b, bo), 0, f[j], R.word(j));
+      else
+        O::st_sr(at_bytes(b, bo), 0, f[j], R.word(j));
     });
   }
 }
@@ -1346,8 +1425,9 @@ __global__ void __launch_bounds__(256) step_kernel(const StepParams p) {
 #pragma unroll
     for (int k = 0; k < NPT; k++) {
       if (act[k]) {
-        collide_node<S, COLL, MODE>(p, bo[k] / (uint32_t)sizeof(S), wall[k], f[k]);
-        store_bulk<S, MODE>(p, bo[k], cidx[k], a[k], wbits[k], f[k]);
+        constexpr bool RAW = raw_store<S, COLL, MODE>();
+        collide_node<S, COLL, MODE, RAW>(p, bo[k] / (uint32_t)sizeof(S), wall[k], f[k]);
+        store_bulk<S, MODE, RAW>(p, bo[k], cidx[k], a[k], wbits[k], f[k]);
       }
     }
   } else {
@@ -1534,15 +1614,30 @@ struct BatchDomain {
   double inv_nxy;
   int32_t has_force;
   int32_t any_wall;      // the solve has obstacles (else no flag / mask is read)
-  float omega_f;
-  float ef3f[Q];
+  float k1;              // FP32 BGK constants (StepParams::k1, kw)
+  float kw[3];
+  float f3[3];           // 3 F (FP32-compute forms)
   float sig[3];          // shifted storage: the solve's rest state (RestShift)
   double tau;
   double inv_tau;
   double ef[Q];
   double cav[2][5];
+  float cavf[2][5];
   double mrt[10];
 };
+
+// A tile is BATCH_TILE_NODES consecutive nodes of one domain, walked by a
+// 256-thread workgroup in passes of 256 nodes: the domain's parameters load
+// once per tile and serve every pass.
+constexpr uint32_t BATCH_TILE_NODES = 1024;
+constexpr uint32_t BATCH_PASS_NODES = 256;
+
+// Work units (one workgroup's share between parameter loads) per tile of the
+// batch kernel for a storage element size and collision: the 16-bit BGK kernel
+// walks a whole tile per unit, the others a 256-node pass.
+HLBM_HD constexpr uint32_t batch_units_per_tile(int esize, int coll) {
+  return (esize == 2 && coll == COLL_BGK) ? 1u : BATCH_TILE_NODES / BATCH_PASS_NODES;
+}
 
 struct BatchTile {
   uint32_t dom;
@@ -1582,8 +1677,11 @@ __device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, c
   p.ln[0] = d.nx;
   p.ln[1] = d.ny;
   p.ln[2] = d.nz;
-  p.fin = (const void*)((const S*)bp.arena[parity] + d.f_off);
-  p.fout = (void*)((S*)bp.arena[parity ^ 1] + d.f_off);
+  // both arena bases are in registers: a select, not an indexed load
+  const S* a0 = (const S*)bp.arena[0];
+  const S* a1 = (const S*)bp.arena[1];
+  p.fin = (const void*)((parity ? a1 : a0) + d.f_off);
+  p.fout = (void*)(const_cast<S*>(parity ? a0 : a1) + d.f_off);
   p.flags = bp.flags + d.flag_off;
   p.m0 = nullptr;
   p.m1 = nullptr;
@@ -1608,13 +1706,16 @@ __device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, c
   for (int k = 0; k < 5; k++) {
     p.cav[0][k] = d.cav[0][k];
     p.cav[1][k] = d.cav[1][k];
+    p.cavf[0][k] = d.cavf[0][k];
+    p.cavf[1][k] = d.cavf[1][k];
   }
   p.tau = d.tau;
   p.inv_tau = d.inv_tau;
   p.has_force = d.has_force;
-  p.omega_f = d.omega_f;
   for (int k = 0; k < 3; k++) p.sig[k] = d.sig[k];
-  for (int i = 0; i < Q; i++) p.ef3f[i] = d.ef3f[i];
+  for (int k = 0; k < 3; k++) p.f3[k] = d.f3[k];
+  p.k1 = d.k1;
+  for (int k = 0; k < 3; k++) p.kw[k] = d.kw[k];
   for (int i = 0; i < Q; i++) p.ef[i] = d.ef[i];
   for (int k = 0; k < 10; k++) p.mrt[k] = d.mrt[k];
   return p;
@@ -1642,33 +1743,40 @@ __device__ __forceinline__ void batch_node(const BatchParams& bp, const BatchDom
   const NodeNbrs a = node_nbrs<S>(p, x, y, z);
   C f[Q];
   gather_select<S, MODE_FUSED>(p, bo, a, wbits, f);
-  collide_node<S, COLL, MODE_FUSED>(p, n, wall, f);
-  store_bulk<S, MODE_FUSED>(p, bo, n, a, wbits, f);  // batch solves: default layout
+  // (solves with obstacles keep the scale in the store: scaling their wall
+  // lanes in the collision would push the kernel past 64 vector registers)
+  constexpr bool RAW = !WALLS && raw_store<S, COLL, MODE_FUSED>();
+  collide_node<S, COLL, MODE_FUSED, RAW>(p, n, wall, f);
+  store_bulk<S, MODE_FUSED, RAW>(p, bo, n, a, wbits, f);  // batch solves: default layout
 }
 
 // One launch per collision model and time step: the BGK launch walks the
 // BGK tiles, the MRT launch the MRT tiles, so each kernel keeps only its own
-// collision in registers.  Tiles are 256 consecutive nodes of one domain, so
-// the domain's parameters are workgroup-uniform.
+// collision in registers.  Tiles are BATCH_TILE_NODES consecutive nodes of one
+// domain, so the domain's parameters are workgroup-uniform.
 //
-// The BGK kernels request the next tile's entry while the current tile's
-// domain parameters load (one scalar-load latency per tile instead of two).
-// With 16-bit storage the BGK kernel also compiles a second node path for
-// solves without obstacles (no mask read) and asks for 8 waves per SIMD, which
-// the register allocator meets without spilling vector registers; the wider
-// kernels and the MRT kernels keep one path, since the second one costs them
-// a wave per SIMD.
+// The 16-bit BGK kernel loads a tile's parameters once and walks its
+// BATCH_TILE_NODES / 256 passes in a loop; it also compiles a second node path
+// for solves without obstacles (no mask read) and asks for 8 waves per SIMD,
+// which the register allocator meets without spilling vector registers.  The
+// wider kernels and the MRT kernels take one 256-node pass per parameter load
+// (units of a quarter tile) and one node path: keeping the parameters live
+// across passes, or a second path, costs them waves per SIMD.  The host sizes
+// each launch to about one workgroup per unit; a workgroup that finds more
+// walks them grid-stride (and the BGK kernels then request the next unit's
+// tile entry while the current one's parameters load).
 template <class S, int COLL>
 __global__ void __launch_bounds__(256, (sizeof(S) == 2 && COLL == COLL_BGK) ? 8 : 1)
     batch_kernel(const BatchParams bp) {
   constexpr bool PREFETCH = COLL == COLL_BGK;
-  constexpr bool TWO_PATHS = sizeof(S) == 2 && COLL == COLL_BGK;
-  uint32_t t0 = COLL == COLL_BGK ? 0u : bp.ntiles[0];
-  uint32_t t1 = COLL == COLL_BGK ? bp.ntiles[0] : bp.ntiles[1];
+  constexpr uint32_t UNITS = batch_units_per_tile((int)sizeof(S), COLL);     // work units per tile
+  constexpr uint32_t INNER = BATCH_TILE_NODES / BATCH_PASS_NODES / UNITS;  // passes per parameter load
+  uint32_t t0 = (COLL == COLL_BGK ? 0u : bp.ntiles[0]) * UNITS;
+  uint32_t t1 = (COLL == COLL_BGK ? bp.ntiles[0] : bp.ntiles[1]) * UNITS;
   uint32_t first = blockIdx.x, stride = gridDim.x;
   if (bp.xcd_split && (gridDim.x & 7u) == 0u) {
     // workgroups go to the 8 XCDs round-robin: XCD k (= blockIdx % 8) takes the
-    // k-th eighth of the tiles, so neighbouring tiles share that XCD's L2
+    // k-th eighth of the units, so neighbouring tiles share that XCD's L2
     const uint32_t k = blockIdx.x & 7u, n = t1 - t0, per = (n + 7u) / 8u;
     const uint32_t a = t0 + min(n, k * per), e = t0 + min(n, (k + 1u) * per);
     t0 = a;
@@ -1676,27 +1784,35 @@ __global__ void __launch_bounds__(256, (sizeof(S) == 2 && COLL == COLL_BGK) ? 8 
     first = blockIdx.x >> 3;
     stride = gridDim.x >> 3;
   }
-  uint32_t t = t0 + first;
+  uint32_t u = t0 + first;
   BatchTile next = BatchTile{0u, 0u};
-  if (PREFETCH && t < t1) next = bp.tiles[t];
-  for (; t < t1; t += stride) {
+  if (PREFETCH && u < t1) next = bp.tiles[u / UNITS];
+  for (; u < t1; u += stride) {
     BatchTile tl;
     if constexpr (PREFETCH) {
       tl = next;
-      if (t + stride < t1) next = bp.tiles[t + stride];
+      if (u + stride < t1) next = bp.tiles[(u + stride) / UNITS];
     } else {
-      tl = bp.tiles[t];
+      tl = bp.tiles[u / UNITS];
     }
     const BatchDomain& d = bp.doms[tl.dom];
     if (bp.step >= d.active_steps) continue;
     const int parity = (d.parity + bp.step) & 1;
     const StepParams p = batch_step_params<S>(bp, d, parity);
-    const uint32_t n = tl.first + threadIdx.x;
-    if (n >= p.N) continue;
-    if (TWO_PATHS && !d.any_wall)
-      batch_node<S, COLL, false>(bp, d, p, n);
-    else
-      batch_node<S, COLL, true>(bp, d, p, n);
+    const uint32_t lo = tl.first + (u % UNITS) * (INNER * BATCH_PASS_NODES);
+    const uint32_t end = min(p.N, lo + INNER * BATCH_PASS_NODES);
+    if constexpr (INNER == 1) {
+      const uint32_t n = lo + threadIdx.x;
+      if (n < end) batch_node<S, COLL, true>(bp, d, p, n);
+    } else {
+#pragma nounroll
+      for (uint32_t n = lo + threadIdx.x; n < end; n += BATCH_PASS_NODES) {
+        if (!d.any_wall)
+          batch_node<S, COLL, false>(bp, d, p, n);
+        else
+          batch_node<S, COLL, true>(bp, d, p, n);
+      }
+    }
   }
 }
 

@@ -19,15 +19,15 @@ hipError_t launch_step_lowp(int storage, int coll, int mode, int npt, dim3 grid,
 }
 
 template <class S>
-static hipError_t batch_pair(unsigned blocks, const BatchParams& bp, hipStream_t s) {
-  hipLaunchKernelGGL((batch_kernel<S, COLL_BGK>), dim3(blocks), dim3(256), 0, s, bp);
+static hipError_t batch_pair(const unsigned blocks[2], const BatchParams& bp, hipStream_t s) {
+  hipLaunchKernelGGL((batch_kernel<S, COLL_BGK>), dim3(blocks[0]), dim3(256), 0, s, bp);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((batch_kernel<S, COLL_MRT>), dim3(blocks), dim3(256), 0, s, bp);
+  hipLaunchKernelGGL((batch_kernel<S, COLL_MRT>), dim3(blocks[1]), dim3(256), 0, s, bp);
   return hipGetLastError();
 }
 
-hipError_t launch_batch_lowp(int storage, unsigned blocks, const BatchParams& bp, hipStream_t s) {
+hipError_t launch_batch_lowp(int storage, const unsigned blocks[2], const BatchParams& bp, hipStream_t s) {
   switch (storage) {
     case 1:
       return batch_pair<float>(blocks, bp, s);
