@@ -107,8 +107,9 @@ struct StepParams {
   float kw[3];          // w_class / tau for the rest, axis and diagonal classes
   float sig[3];         // shifted storage: rest state w_i + sig[class(i)] (RestShift)
   float f3[3];          // 3 F for the FP32-compute forms (the force term is rho w_i c_i . 3F)
-  double cav[2][5];
   float cavf[2][5];     // cav in FP32 for the FP32-compute forms (uniform registers, no conversions)
+  float pad_[3];        // keeps the FP64 fields below at the offsets the FP64 kernels were tuned with
+  double cav[2][5];
   double tau;
   double inv_tau;       // 1 / tau (the MRT default s9 = s13, mrt.hpp:43)
   double F[3];
@@ -1801,13 +1802,19 @@ __global__ void __launch_bounds__(256, (sizeof(S) == 2 && COLL == COLL_BGK) ? 8 
     const StepParams p = batch_step_params<S>(bp, d, parity);
     const uint32_t lo = tl.first + (u % UNITS) * (INNER * BATCH_PASS_NODES);
     const uint32_t end = min(p.N, lo + INNER * BATCH_PASS_NODES);
+    constexpr bool TWO_PATHS = sizeof(S) == 2 && COLL == COLL_BGK;
     if constexpr (INNER == 1) {
       const uint32_t n = lo + threadIdx.x;
-      if (n < end) batch_node<S, COLL, true>(bp, d, p, n);
+      if (n < end) {
+        if (TWO_PATHS && !d.any_wall)
+          batch_node<S, COLL, false>(bp, d, p, n);
+        else
+          batch_node<S, COLL, true>(bp, d, p, n);
+      }
     } else {
 #pragma nounroll
       for (uint32_t n = lo + threadIdx.x; n < end; n += BATCH_PASS_NODES) {
-        if (!d.any_wall)
+        if (TWO_PATHS && !d.any_wall)
           batch_node<S, COLL, false>(bp, d, p, n);
         else
           batch_node<S, COLL, true>(bp, d, p, n);
