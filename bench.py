@@ -99,9 +99,9 @@ def porous_channel(steps=100, warmup=10, storage="f64", tune=True):
 
 def batched_cavities(n_dom=2048, chunk=200, seed=4, storage="f16", use_graph=True):
     """2048 independent lid-driven cavities, edges ~U{16..96} per axis, lid speed
-    ~U[0.01,0.1], tau ~U[0.52,1.0], BGK, one persistent launch per step, chunks
-    of `chunk` steps replayed from a captured HIP graph (use_graph=False: the
-    same launches issued one by one on the stream)."""
+    ~U[0.01,0.1], tau ~U[0.52,1.0], BGK, one launch per step (one workgroup per
+    1024-node tile), chunks of `chunk` steps replayed from a captured HIP graph
+    (use_graph=False: the same launches issued one by one on the stream)."""
     import numpy as np
     from hippolbm_amd.batch import Batch, Solve
     rng = np.random.default_rng(seed)
@@ -364,6 +364,9 @@ def main():
     ap.add_argument("--batch-launch", choices=("graph", "stream"), default="graph",
                     help="batched and served legs: replay each chunk from a HIP graph, or launch its steps "
                          "one by one on the stream (per-dispatch profiling)")
+    ap.add_argument("--batch-domains", type=int, default=2048,
+                    help="batched leg: the first N cavities of the seeded mix (counter passes use a few hundred)")
+    ap.add_argument("--batch-chunk", type=int, default=200, help="batched leg: steps per chunk")
     ap.add_argument("--precision", type=int, default=0, metavar="STEPS",
                     help="run only the precision sweep (Couette 512^3, Poiseuille 512x256x256) for STEPS steps")
     ap.add_argument("--no-tune", action="store_true", help="keep the default launch shape (no autotuning)")
@@ -395,7 +398,7 @@ def main():
         g = args.batch_launch == "graph"
         out = {"batch_launch": args.batch_launch}
         if args.batch_only in ("both", "batched"):
-            out["batched"] = batched_cavities(use_graph=g)
+            out["batched"] = batched_cavities(n_dom=args.batch_domains, chunk=args.batch_chunk, use_graph=g)
         if args.batch_only in ("both", "served"):
             out["served"] = served_sweep(use_graph=g)
         print(json.dumps(out), flush=True)
