@@ -151,7 +151,6 @@ static StepParams base_params(hlbm_domain* h) {
   for (int d = 0; d < 3; d++) p.F[d] = h->d.fext[d];
   // c_i . F in the reference's order: ex*Fx + ey*Fy + ez*Fz (bgk.hpp:57, mrt.hpp:188)
   for (int i = 0; i < Q; i++) p.ef[i] = CX(i) * h->d.fext[0] + CY(i) * h->d.fext[1] + CZ(i) * h->d.fext[2];
-  set_fp32_constants(p);
   {
     const RestShift sh = dom_shift(h);
     for (int k = 0; k < 3; k++) p.sig[k] = (float)sh.s[k];
@@ -164,6 +163,7 @@ static StepParams base_params(hlbm_domain* h) {
   p.gz = along(h, 2);
   p.xcd_remap = h->xcd_remap;
   for (int k = 0; k < 10; k++) p.mrt[k] = h->d.mrt_rates[k];
+  set_fp32_constants(p);  // after tau, c_i . F, the cavity terms and the MRT rates
   if (!h->any_wall) p.wall_bb = 0;
   p.any_wall = h->any_wall ? 1 : 0;
   p.bbm = h->bbm;

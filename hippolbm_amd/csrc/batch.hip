@@ -185,8 +185,8 @@ static StepParams single_params(hlbm_batch* b, const Solve& s, int parity) {
   p.inv_tau = s.dom.inv_tau;
   for (int k = 0; k < 3; k++) p.sig[k] = s.dom.sig[k];
   for (int i = 0; i < Q; i++) p.ef[i] = s.dom.ef[i];
-  set_fp32_constants(p);
   for (int k = 0; k < 10; k++) p.mrt[k] = s.dom.mrt[k];
+  set_fp32_constants(p);
   p.parts = PART_ALL;
   p.gx = (s.dom.nx + 127) / 128;
   p.gy = (s.dom.ny + 1) / 2;
@@ -414,6 +414,7 @@ int hlbm_batch_submit(hlbm_batch* b, const hlbm_solve_desc* s, int64_t* id) {
     for (int k = 0; k < 3; k++) d.sig[k] = (float)sh.s[k];
   }
   for (int k = 0; k < 10; k++) d.mrt[k] = s->mrt_rates[k];
+  mrt_fp32_rates(d.mrt_custom, d.mrt, d.inv_tau, d.mrtf);
   const double L = d.nz;  // cavity.hpp:58-63
   for (int side = 0; side < 2; side++) {
     d.cav_on[side] = s->cavity_on[side] ? 1 : 0;

@@ -53,6 +53,17 @@ inline void bgk_fp32_constants(double tau, float& k1, float kw[3]) {
   k1 = (float)(1.0 - 18.0 * (double)best);
 }
 
+// FP32 MRT rates s1 s2 s4 s9 s10 s13 s16 weps wepsj wxx: the custom ones, or
+// hippoLBM's defaults (mrt.hpp:43-47) with s9 = s13 = 1/tau.
+inline void mrt_fp32_rates(int custom, const double* mrt, double inv_tau, float out[10]) {
+  if (custom) {
+    for (int k = 0; k < 10; k++) out[k] = (float)mrt[k];
+    return;
+  }
+  const double d[10] = {1.19, 1.4, 1.2, inv_tau, 1.4, inv_tau, 1.98, 0.0, -475.0 / 63.0, 0.0};
+  for (int k = 0; k < 10; k++) out[k] = (float)d[k];
+}
+
 // c_1, c_3, c_5 are the unit vectors +x, +y, +z, so ef[1], ef[3], ef[5] are F.
 static_assert(CX(1) == 1 && CY(3) == 1 && CZ(5) == 1, "axis directions");
 inline void set_fp32_constants(StepParams& p) {
@@ -63,6 +74,7 @@ inline void set_fp32_constants(StepParams& p) {
   p.f3[1] = 3.0f * (float)p.ef[3];
   p.f3[2] = 3.0f * (float)p.ef[5];
   bgk_fp32_constants(p.tau, p.k1, p.kw);
+  mrt_fp32_rates(p.mrt_custom, p.mrt, p.inv_tau, p.mrtf);
   for (int side = 0; side < 2; side++)
     for (int j = 0; j < 5; j++) p.cavf[side][j] = (float)p.cav[side][j];
 }

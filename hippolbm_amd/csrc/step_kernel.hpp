@@ -31,6 +31,7 @@
 #include <type_traits>
 
 #include "d3q19.hpp"
+#include "mrt_pairs.hpp"
 
 namespace hlbm {
 
@@ -126,6 +127,7 @@ struct StepParams {
   int32_t nset;                // start_iteration set_distribution boxes
   int32_t set_box[MAX_SET][6]; // node box, inclusive: x0 y0 z0 x1 y1 z1
   double set_val[MAX_SET];     // f_i = value * w_i
+  float mrtf[10];              // the MRT rates in FP32 (defaults or custom), for the FP32-compute forms
 };
 
 // ---------------------------------------------------------------------------
@@ -419,72 +421,7 @@ __device__ __forceinline__ float recip_f32(float x) {
 // direction order and each inverse row in moment order, as the reference does.
 // moment order: rho e eps jx qx jy qy jz qz pxx3 pixx3 pww piww pxy pyz pxz mx my mz
 // ---------------------------------------------------------------------------
-struct MrtBasis {
-  static constexpr int M[19][19] = {
-      {1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
-      {-30, -11, -11, -11, -11, -11, -11, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8, 8},
-      {12, -4, -4, -4, -4, -4, -4, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
-      {0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0},
-      {0, -4, 4, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0},
-      {0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1},
-      {0, 0, 0, -4, 4, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1},
-      {0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1},
-      {0, 0, 0, 0, 0, -4, 4, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1},
-      {0, 2, 2, -1, -1, -1, -1, 1, 1, 1, 1, 1, 1, 1, 1, -2, -2, -2, -2},
-      {0, -4, -4, 2, 2, 2, 2, 1, 1, 1, 1, 1, 1, 1, 1, -2, -2, -2, -2},
-      {0, 0, 0, 1, 1, -1, -1, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0},
-      {0, 0, 0, -2, -2, 2, 2, 1, 1, 1, 1, -1, -1, -1, -1, 0, 0, 0, 0},
-      {0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0, 0, 0, 0, 0},
-      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1},
-      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, -1, -1, 0, 0, 0, 0},
-      {0, 0, 0, 0, 0, 0, 0, 1, -1, 1, -1, -1, 1, -1, 1, 0, 0, 0, 0},
-      {0, 0, 0, 0, 0, 0, 0, -1, 1, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1},
-      {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, 1, -1},
-  };
-  // inverse transform: f_i = sum_k sign*num/den * m_k; stored as signed numerator and denominator
-  static constexpr int NUM[19][19] = {
-      {1, -5, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0},
-      {1, -11, -1, 1, -1, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 0, 0, 0, 0},
-      {1, -11, -1, -1, 1, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 0, 0, 0, 0},
-      {1, -11, -1, 0, 0, 1, -1, 0, 0, -1, 1, 1, -1, 0, 0, 0, 0, 0, 0},
-      {1, -11, -1, 0, 0, -1, 1, 0, 0, -1, 1, 1, -1, 0, 0, 0, 0, 0, 0},
-      {1, -11, -1, 0, 0, 0, 0, 1, -1, -1, 1, -1, 1, 0, 0, 0, 0, 0, 0},
-      {1, -11, -1, 0, 0, 0, 0, -1, 1, -1, 1, -1, 1, 0, 0, 0, 0, 0, 0},
-      {1, 4, 1, 1, 1, 1, 1, 0, 0, 1, 1, 1, 1, 1, 0, 0, 1, -1, 0},
-      {1, 4, 1, -1, -1, -1, -1, 0, 0, 1, 1, 1, 1, 1, 0, 0, -1, 1, 0},
-      {1, 4, 1, 1, 1, -1, -1, 0, 0, 1, 1, 1, 1, -1, 0, 0, 1, 1, 0},
-      {1, 4, 1, -1, -1, 1, 1, 0, 0, 1, 1, 1, 1, -1, 0, 0, -1, -1, 0},
-      {1, 4, 1, 1, 1, 0, 0, 1, 1, 1, 1, -1, -1, 0, 0, 1, -1, 0, 1},
-      {1, 4, 1, -1, -1, 0, 0, -1, -1, 1, 1, -1, -1, 0, 0, 1, 1, 0, -1},
-      {1, 4, 1, 1, 1, 0, 0, -1, -1, 1, 1, -1, -1, 0, 0, -1, -1, 0, -1},
-      {1, 4, 1, -1, -1, 0, 0, 1, 1, 1, 1, -1, -1, 0, 0, -1, 1, 0, 1},
-      {1, 4, 1, 0, 0, 1, 1, 1, 1, -1, -1, 0, 0, 0, 1, 0, 0, 1, -1},
-      {1, 4, 1, 0, 0, -1, -1, -1, -1, -1, -1, 0, 0, 0, 1, 0, 0, -1, 1},
-      {1, 4, 1, 0, 0, 1, 1, -1, -1, -1, -1, 0, 0, 0, -1, 0, 0, 1, 1},
-      {1, 4, 1, 0, 0, -1, -1, 1, 1, -1, -1, 0, 0, 0, -1, 0, 0, -1, -1},
-  };
-  static constexpr int DEN[19][19] = {
-      {19, 399, 21, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1},
-      {19, 2394, 63, 10, 10, 1, 1, 1, 1, 18, 18, 1, 1, 1, 1, 1, 1, 1, 1},
-      {19, 2394, 63, 10, 10, 1, 1, 1, 1, 18, 18, 1, 1, 1, 1, 1, 1, 1, 1},
-      {19, 2394, 63, 1, 1, 10, 10, 1, 1, 36, 36, 12, 12, 1, 1, 1, 1, 1, 1},
-      {19, 2394, 63, 1, 1, 10, 10, 1, 1, 36, 36, 12, 12, 1, 1, 1, 1, 1, 1},
-      {19, 2394, 63, 1, 1, 1, 1, 10, 10, 36, 36, 12, 12, 1, 1, 1, 1, 1, 1},
-      {19, 2394, 63, 1, 1, 1, 1, 10, 10, 36, 36, 12, 12, 1, 1, 1, 1, 1, 1},
-      {19, 1197, 252, 10, 40, 10, 40, 1, 1, 36, 72, 12, 24, 4, 1, 1, 8, 8, 1},
-      {19, 1197, 252, 10, 40, 10, 40, 1, 1, 36, 72, 12, 24, 4, 1, 1, 8, 8, 1},
-      {19, 1197, 252, 10, 40, 10, 40, 1, 1, 36, 72, 12, 24, 4, 1, 1, 8, 8, 1},
-      {19, 1197, 252, 10, 40, 10, 40, 1, 1, 36, 72, 12, 24, 4, 1, 1, 8, 8, 1},
-      {19, 1197, 252, 10, 40, 1, 1, 10, 40, 36, 72, 12, 24, 1, 1, 4, 8, 1, 8},
-      {19, 1197, 252, 10, 40, 1, 1, 10, 40, 36, 72, 12, 24, 1, 1, 4, 8, 1, 8},
-      {19, 1197, 252, 10, 40, 1, 1, 10, 40, 36, 72, 12, 24, 1, 1, 4, 8, 1, 8},
-      {19, 1197, 252, 10, 40, 1, 1, 10, 40, 36, 72, 12, 24, 1, 1, 4, 8, 1, 8},
-      {19, 1197, 252, 1, 1, 10, 40, 10, 40, 18, 36, 1, 1, 1, 4, 1, 1, 8, 8},
-      {19, 1197, 252, 1, 1, 10, 40, 10, 40, 18, 36, 1, 1, 1, 4, 1, 1, 8, 8},
-      {19, 1197, 252, 1, 1, 10, 40, 10, 40, 18, 36, 1, 1, 1, 4, 1, 1, 8, 8},
-      {19, 1197, 252, 1, 1, 10, 40, 10, 40, 18, 36, 1, 1, 1, 4, 1, 1, 8, 8},
-  };
-};
+// MrtBasis (the matrix, its exact inverse as numerators/denominators): mrt_pairs.hpp
 
 constexpr bool pow2(int v) { return v > 0 && (v & (v - 1)) == 0; }
 
@@ -558,6 +495,14 @@ __device__ __forceinline__ C inverse_row(const C* mo) {
 struct MrtRates {
   double s1, s2, s4, s9, s10, s13, s16, weps, wepsj, wxx;
 };
+struct MrtRatesF {
+  float s1, s2, s4, s9, s10, s13, s16, weps, wepsj, wxx;
+};
+
+__device__ __forceinline__ MrtRatesF mrt_rates_f32(const StepParams& p) {
+  const float* r = p.mrtf;
+  return MrtRatesF{r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7], r[8], r[9]};
+}
 
 __device__ __forceinline__ MrtRates mrt_rates(const StepParams& p) {
   MrtRates r;
@@ -591,8 +536,8 @@ __device__ __forceinline__ MrtRates mrt_rates(const StepParams& p) {
 // caller passes the moments of g = f - w; the rest state w has the moments
 // rho = 1, e = -11, eps = 3 (all others 0), which enter through rest_e/rest_eps
 // and rest_rho (all zero for the unshifted FP64 path).
-template <class C>
-__device__ __forceinline__ void mrt_relax(C* m, const MrtRates& r, C rest_rho, C rest_e, C rest_eps) {
+template <class C, class R>
+__device__ __forceinline__ void mrt_relax(C* m, const R& r, C rest_rho, C rest_e, C rest_eps) {
   const C rho = m[0] + rest_rho;
   const C e = m[1], eps = m[2], jx = m[3], qx = m[4], jy = m[5], qy = m[6], jz = m[7], qz = m[8];
   const C pxx3 = m[9], pixx3 = m[10], pww = m[11], piww = m[12], pxy = m[13], pyz = m[14], pxz = m[15];
@@ -628,27 +573,46 @@ __device__ __forceinline__ void mrt_relax(C* m, const MrtRates& r, C rest_rho, C
 template <class C>
 __device__ __forceinline__ void mrt_collide(C* f, C rho_force, const StepParams& p, bool shifted) {
   C m[19];
-  static_for([&](auto kc) {
-    constexpr int k = decltype(kc)::value;
-    m[k] = moment<k>(f);
-  });
-  const MrtRates r = mrt_rates(p);
-  // shifted storage holds g = f - (w + sig): the rest state's moments are
-  // rho = 1, e = -11 and eps = weps (RestShift)
-  if (shifted)
-    mrt_relax<C>(m, r, C(1), C(-11), C(r.weps));
-  else
-    mrt_relax<C>(m, r, C(0), C(0), C(0));
-  static_for([&](auto ic) {
-    constexpr int i = decltype(ic)::value;
-    f[i] = inverse_row<i>(m);
-  });
-  // mrt.hpp:187-191, force term with rho from macro_variables (exactly +0 when F = 0)
-  if (p.has_force) {
+  if constexpr (std::is_same<C, double>::value) {
+    static_for([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      m[k] = moment<k>(f);
+    });
+    const MrtRates r = mrt_rates(p);
+    // shifted storage holds g = f - (w + sig): the rest state's moments are
+    // rho = 1, e = -11 and eps = weps (RestShift)
+    if (shifted)
+      mrt_relax<C>(m, r, C(1), C(-11), C(r.weps));
+    else
+      mrt_relax<C>(m, r, C(0), C(0), C(0));
     static_for([&](auto ic) {
       constexpr int i = decltype(ic)::value;
-      f[i] += C(3.) * rho_force * C(WD(i)) * C(p.ef[i]);
+      f[i] = inverse_row<i>(m);
     });
+    // mrt.hpp:187-191, force term with rho from macro_variables (exactly +0 when F = 0)
+    if (p.has_force) {
+      static_for([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        f[i] += C(3.) * rho_force * C(WD(i)) * C(p.ef[i]);
+      });
+    }
+  } else {
+    // FP32: the pair forms of the transform and FP32 rates (same basis and
+    // relaxation, far fewer operations); the force term from 3F as for BGK
+    mrt_moments_pairs(f, m);
+    const MrtRatesF r = mrt_rates_f32(p);
+    if (shifted)
+      mrt_relax<C>(m, r, C(1), C(-11), C(r.weps));
+    else
+      mrt_relax<C>(m, r, C(0), C(0), C(0));
+    mrt_inverse_pairs(m, f);
+    if (p.has_force) {
+      const float fx = p.f3[0], fy = p.f3[1], fz = p.f3[2];
+      static_for([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        f[i] += (rho_force * WF(i)) * cdot<i>(fx, fy, fz);
+      });
+    }
   }
 }
 
@@ -1625,6 +1589,7 @@ struct BatchDomain {
   double cav[2][5];
   float cavf[2][5];
   double mrt[10];
+  float mrtf[10];        // StepParams::mrtf
 };
 
 // A tile is BATCH_TILE_NODES consecutive nodes of one domain, walked by a
@@ -1719,6 +1684,7 @@ __device__ __forceinline__ StepParams batch_step_params(const BatchParams& bp, c
   for (int k = 0; k < 3; k++) p.kw[k] = d.kw[k];
   for (int i = 0; i < Q; i++) p.ef[i] = d.ef[i];
   for (int k = 0; k < 10; k++) p.mrt[k] = d.mrt[k];
+  for (int k = 0; k < 10; k++) p.mrtf[k] = d.mrtf[k];
   return p;
 }
 
