@@ -22,10 +22,14 @@ pytestmark = pytest.mark.gpu
 
 SWEEP_STEPS = 10000
 
-# relative L2 of u against FP64 after 10,000 steps (bounds asserted below);
-# measured on MI355X: see the module docstring of each test
-U_L2_BOUND = {"f32": 2e-4, "f16": 1e-3, "bf16": 3e-3}
-RHO_MAX_BOUND = {"f32": 1e-6, "f16": 5e-5, "bf16": 5e-4}
+# relative L2 of u and max |rho error| against FP64 after 10,000 steps.  Measured
+# on MI355X (Couette / Poiseuille; the arithmetic is deterministic, stochastic
+# rounding included, so the figures repeat): FP32 4.7e-7 / 7.3e-5 and
+# 4.4e-13 / 9.5e-13, FP16 1.5e-4 / 2.4e-4 and 4.1e-5 / 8.6e-6, BF16
+# 1.2e-3 / 1.5e-3 and 3.3e-4 / 6.8e-5.  The FP32 and FP16 bounds are the
+# targets (1e-4, 1e-3); BF16 carries 8 bits of mantissa.
+U_L2_BOUND = {"f32": 1e-4, "f16": 1e-3, "bf16": 3e-3}
+RHO_MAX_BOUND = {"f32": 1e-11, "f16": 5e-5, "bf16": 5e-4}
 
 
 def say(*a):
