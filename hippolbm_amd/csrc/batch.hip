@@ -136,6 +136,8 @@ struct hlbm_batch {
   unsigned grid[2] = {8u, 8u};  // workgroups of the BGK and MRT launches (graph captured with these)
   hipStream_t stream = nullptr;       // chunks, admission, retirement
   hipStream_t copy_stream = nullptr;  // result read-back
+  hipStream_t side = nullptr;         // the MRT solves' chain of a chunk (the BGK chain runs on `stream`)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   hipGraphExec_t graph = nullptr;
   struct ChunkEvents {
     hipEvent_t start = nullptr, end = nullptr;
@@ -203,7 +205,7 @@ static hipError_t single_launch(hlbm_batch* b, const Solve& s, int mode, int par
   return launch_step_lowp(b->d.storage, coll, mode, 1, grid, block, p, b->stream);
 }
 
-static hipError_t chunk_launch(hlbm_batch* b, int step) {
+static hipError_t chunk_launch(hlbm_batch* b, int coll, int step, hipStream_t st) {
   BatchParams bp;
   bp.doms = b->d_doms;
   bp.tiles = b->d_tiles;
@@ -215,8 +217,25 @@ static hipError_t chunk_launch(hlbm_batch* b, int step) {
   bp.step = step;
   bp.xcd_split = b->xcd_split;
   bp.sr = b->sr;
-  if (b->d.storage == HLBM_F64) return launch_batch_f64(b->grid, bp, b->stream);
-  return launch_batch_lowp(b->d.storage, b->grid, bp, b->stream);
+  if (b->d.storage == HLBM_F64) return launch_batch_f64(coll, b->grid[coll], bp, st);
+  return launch_batch_lowp(b->d.storage, coll, b->grid[coll], bp, st);
+}
+
+// The steps of a chunk.  BGK and MRT solves never share a domain, so the two
+// collision models run as independent chains, BGK on `stream` and MRT on the
+// side stream, joined only at the end: each chain's per-step tail (its last
+// workgroups) overlaps the other chain's work.  Both streams have the default
+// priority.
+static hipError_t chunk_sequence(hlbm_batch* b, int nsteps) {
+  hipError_t e = hipEventRecord(b->ev_fork, b->stream);
+  if (e == hipSuccess) e = hipStreamWaitEvent(b->side, b->ev_fork, 0);
+  for (int k = 0; k < nsteps && e == hipSuccess; k++) {
+    e = chunk_launch(b, COLL_BGK, k, b->stream);
+    if (e == hipSuccess) e = chunk_launch(b, COLL_MRT, k, b->side);
+  }
+  if (e == hipSuccess) e = hipEventRecord(b->ev_join, b->side);
+  if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->ev_join, 0);
+  return e;
 }
 
 // Launch sizes follow the work: one workgroup per work unit (a persistent grid
@@ -243,12 +262,10 @@ static int build_graph(hlbm_batch* b) {
   hipGraph_t g = nullptr;
   if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return bfail(HLBM_ERR_HIP, "graph capture begin failed");
-  for (int k = 0; k < b->d.chunk_steps; k++) {
-    if (chunk_launch(b, k) != hipSuccess) {
-      (void)hipStreamEndCapture(b->stream, &g);
-      if (g) (void)hipGraphDestroy(g);
-      return bfail(HLBM_ERR_HIP, "batch kernel launch failed during capture");
-    }
+  if (chunk_sequence(b, b->d.chunk_steps) != hipSuccess) {
+    (void)hipStreamEndCapture(b->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    return bfail(HLBM_ERR_HIP, "batch kernel launch failed during capture");
   }
   if (hipStreamEndCapture(b->stream, &g) != hipSuccess) return bfail(HLBM_ERR_HIP, "graph capture end failed");
   hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
@@ -306,8 +323,12 @@ int hlbm_batch_create(const hlbm_batch_desc* desc, hlbm_batch** out) {
       return cleanup("events");
     b->h_ntiles_set[k][0] = b->h_ntiles_set[k][1] = 0;
   }
-  if (hipStreamCreate(&b->stream) != hipSuccess || hipStreamCreate(&b->copy_stream) != hipSuccess)
+  if (hipStreamCreate(&b->stream) != hipSuccess || hipStreamCreate(&b->copy_stream) != hipSuccess ||
+      hipStreamCreate(&b->side) != hipSuccess)
     return cleanup("stream");
+  if (hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) != hipSuccess)
+    return cleanup("events");
   if (hipMemcpy(b->d_ntiles, b->h_ntiles_set[0], 2 * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
     return cleanup("tile count init");
   b->fl_pop.init(b->arena_elems);
@@ -327,6 +348,7 @@ int hlbm_batch_create(const hlbm_batch_desc* desc, hlbm_batch** out) {
 int hlbm_batch_destroy(hlbm_batch* b) {
   if (!b) return HLBM_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->side) (void)hipStreamSynchronize(b->side);
   if (b->copy_stream) (void)hipStreamSynchronize(b->copy_stream);
   if (b->graph) (void)hipGraphExecDestroy(b->graph);
   for (void* p : {b->arena[0], b->arena[1], (void*)b->flags, (void*)b->bbm, (void*)b->d_doms, (void*)b->d_tiles,
@@ -340,7 +362,10 @@ int hlbm_batch_destroy(hlbm_batch* b) {
   }
   for (auto& kv : b->solves)
     if (kv.second.ev) (void)hipEventDestroy(kv.second.ev);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
   if (b->stream) (void)hipStreamDestroy(b->stream);
+  if (b->side) (void)hipStreamDestroy(b->side);
   if (b->copy_stream) (void)hipStreamDestroy(b->copy_stream);
   delete b;
   return HLBM_OK;
@@ -690,8 +715,7 @@ int hlbm_batch_run(hlbm_batch* b, int32_t* n_finished) {
       if (b->graph && steps_needed == K) {
         if (hipGraphLaunch(b->graph, st) != hipSuccess) return bfail(HLBM_ERR_HIP, "graph launch");
       } else {
-        for (int k = 0; k < steps_needed; k++)
-          if (chunk_launch(b, k) != hipSuccess) return bfail(HLBM_ERR_HIP, "batch kernel launch");
+        if (chunk_sequence(b, steps_needed) != hipSuccess) return bfail(HLBM_ERR_HIP, "batch kernel launch");
       }
     }
     if (hipEventRecord(c.end, st) != hipSuccess) return bfail(HLBM_ERR_HIP, "event");

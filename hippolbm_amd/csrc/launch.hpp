@@ -169,10 +169,10 @@ hipError_t launch_step_f64(int coll, int mode, int npt, dim3 grid, dim3 block, c
 hipError_t launch_step_lowp(int storage, int coll, int mode, int npt, dim3 grid, dim3 block, const StepParams& p,
                             hipStream_t s);
 
-// batched domains, one launch per collision model and step; blocks[0],
-// blocks[1]: workgroups of the BGK and the MRT launch (any number is correct:
-// a workgroup walks units grid-stride)
-hipError_t launch_batch_f64(const unsigned blocks[2], const BatchParams& bp, hipStream_t s);
-hipError_t launch_batch_lowp(int storage, const unsigned blocks[2], const BatchParams& bp, hipStream_t s);
+// batched domains: one launch of the BGK or the MRT kernel for one step with
+// `blocks` workgroups (any number is correct: a workgroup walks units
+// grid-stride)
+hipError_t launch_batch_f64(int coll, unsigned blocks, const BatchParams& bp, hipStream_t s);
+hipError_t launch_batch_lowp(int storage, int coll, unsigned blocks, const BatchParams& bp, hipStream_t s);
 
 }  // namespace hlbm

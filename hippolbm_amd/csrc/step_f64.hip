@@ -9,11 +9,11 @@ hipError_t launch_step_f64(int coll, int mode, int npt, dim3 grid, dim3 block, c
   return step_dispatch<double>(coll, mode, npt, grid, block, p, s);
 }
 
-hipError_t launch_batch_f64(const unsigned blocks[2], const BatchParams& bp, hipStream_t s) {
-  hipLaunchKernelGGL((batch_kernel<double, COLL_BGK>), dim3(blocks[0]), dim3(256), 0, s, bp);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((batch_kernel<double, COLL_MRT>), dim3(blocks[1]), dim3(256), 0, s, bp);
+hipError_t launch_batch_f64(int coll, unsigned blocks, const BatchParams& bp, hipStream_t s) {
+  if (coll == COLL_BGK)
+    hipLaunchKernelGGL((batch_kernel<double, COLL_BGK>), dim3(blocks), dim3(256), 0, s, bp);
+  else
+    hipLaunchKernelGGL((batch_kernel<double, COLL_MRT>), dim3(blocks), dim3(256), 0, s, bp);
   return hipGetLastError();
 }
 

@@ -19,22 +19,22 @@ hipError_t launch_step_lowp(int storage, int coll, int mode, int npt, dim3 grid,
 }
 
 template <class S>
-static hipError_t batch_pair(const unsigned blocks[2], const BatchParams& bp, hipStream_t s) {
-  hipLaunchKernelGGL((batch_kernel<S, COLL_BGK>), dim3(blocks[0]), dim3(256), 0, s, bp);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((batch_kernel<S, COLL_MRT>), dim3(blocks[1]), dim3(256), 0, s, bp);
+static hipError_t batch_one(int coll, unsigned blocks, const BatchParams& bp, hipStream_t s) {
+  if (coll == COLL_BGK)
+    hipLaunchKernelGGL((batch_kernel<S, COLL_BGK>), dim3(blocks), dim3(256), 0, s, bp);
+  else
+    hipLaunchKernelGGL((batch_kernel<S, COLL_MRT>), dim3(blocks), dim3(256), 0, s, bp);
   return hipGetLastError();
 }
 
-hipError_t launch_batch_lowp(int storage, const unsigned blocks[2], const BatchParams& bp, hipStream_t s) {
+hipError_t launch_batch_lowp(int storage, int coll, unsigned blocks, const BatchParams& bp, hipStream_t s) {
   switch (storage) {
     case 1:
-      return batch_pair<float>(blocks, bp, s);
+      return batch_one<float>(coll, blocks, bp, s);
     case 2:
-      return batch_pair<_Float16>(blocks, bp, s);
+      return batch_one<_Float16>(coll, blocks, bp, s);
     case 3:
-      return batch_pair<bf16s>(blocks, bp, s);
+      return batch_one<bf16s>(coll, blocks, bp, s);
     default:
       return hipErrorInvalidValue;
   }
