@@ -1,8 +1,8 @@
 """Batched domains and the solve server (``hlbm_batch_*`` in include/hlbm.h).
 
 ``Batch`` owns an HBM budget in which many independent solves of uneven sizes
-run together: one persistent kernel launch per time step over all running
-solves, chunks of ``chunk_steps`` replayed from a captured HIP graph, solves
+run together: one kernel launch per collision model and time step over all
+running solves (one workgroup per 1024-node tile), chunks of ``chunk_steps`` replayed from a captured HIP graph, solves
 admitted as memory frees up and retired (rho/u copied out) as they finish.
 ``serve`` drives a Poisson stream of requests through it and reports
 time-to-result percentiles beside the aggregate MLUPS.  A solve whose final

@@ -256,10 +256,11 @@ int hlbm_line_velocity(hlbm_domain* h, const int32_t start[3], int32_t axis, int
 int hlbm_ghost_update_ref(const hlbm_domain* h, double* f_ref);
 
 /* ---- batched domains and the solve server -------------------------------
- * Many independent domains of uneven sizes advance together: one persistent
- * kernel launch per time step walks a tile table covering every running
- * domain, with per-domain tau / force / boundary tables.  Chunks of
- * `chunk_steps` steps are captured once as a HIP graph and replayed.
+ * Many independent domains of uneven sizes advance together: one kernel launch
+ * per collision model and time step covers a tile table of every running
+ * domain (one workgroup per 1024-node tile), with per-domain tau / force /
+ * boundary tables.  Chunks of `chunk_steps` steps are captured as a HIP graph
+ * and replayed (captured again when the tile count outgrows the launch).
  * Requests ("solves") are submitted at any time; each hlbm_batch_run() call
  * admits queued solves whose populations fit the HBM budget, runs one chunk,
  * and retires finished solves (their rho/u are copied to the host and their
