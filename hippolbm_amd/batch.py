@@ -2,8 +2,9 @@
 
 ``Batch`` owns an HBM budget in which many independent solves of uneven sizes
 run together: one kernel launch per collision model and time step over all
-running solves (one workgroup per 1024-node tile), chunks of ``chunk_steps`` replayed from a captured HIP graph, solves
-admitted as memory frees up and retired (rho/u copied out) as they finish.
+running solves (one workgroup per 1024-node tile), chunks of ``chunk_steps``
+replayed from a captured HIP graph, solves admitted as memory frees up and
+retired (rho/u copied out) as they finish.
 ``serve`` drives a Poisson stream of requests through it and reports
 time-to-result percentiles beside the aggregate MLUPS.  A solve whose final
 state is not finite ends FAILED (its own status; the other solves are not
