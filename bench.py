@@ -124,7 +124,8 @@ def batched_cavities(n_dom=2048, chunk=200, seed=4, storage="f16", use_graph=Tru
     mlups = nodes_total * chunk / (ms * 1e-3) / 1e6
     bpn = 2 * 19 * BYTES[storage] + 1
     return {"domains": n_dom, "nodes": nodes_total, "chunk_steps": chunk, "storage": storage, "mlups": mlups,
-            "ms_per_step": ms / chunk, "frac": mlups * 1e6 * bpn / 1e9 / HBM_PEAK_GBS}
+            "ms_per_step": ms / chunk, "frac": mlups * 1e6 * bpn / 1e9 / HBM_PEAK_GBS,
+            "arenas_relocated": i1["relocated"]}
 
 
 # arrival rates of the served legs: 8 Hz loads the FP32 path to about 85 % of its
@@ -157,6 +158,7 @@ def served_stream(n=SERVED_N, rate_hz=8.0, seed=5, storage="f16", chunk=200, bud
     reqs = poisson_requests(n, rate_hz, seed)
     b = Batch(storage=storage, budget_bytes=budget_gb << 30, chunk_steps=chunk, use_graph=use_graph)
     r = serve(b, reqs)
+    r["arenas_relocated"] = b.info()["relocated"]
     b.close()
     bpn = 2 * 19 * BYTES[storage] + 1
     r.pop("results")

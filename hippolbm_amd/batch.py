@@ -47,7 +47,7 @@ class BatchInfo(ctypes.Structure):
     _fields_ = [("queued", ctypes.c_int64), ("running", ctypes.c_int64), ("done", ctypes.c_int64),
                 ("chunks", ctypes.c_int64), ("bytes_in_use", ctypes.c_uint64), ("budget_bytes", ctypes.c_uint64),
                 ("node_updates", ctypes.c_double), ("gpu_ms", ctypes.c_double),
-                ("peak_bytes_in_use", ctypes.c_uint64)]
+                ("peak_bytes_in_use", ctypes.c_uint64), ("relocated", ctypes.c_int64)]
 
 
 _bound = False

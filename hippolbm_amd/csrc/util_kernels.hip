@@ -495,6 +495,25 @@ __global__ void __launch_bounds__(256) hbm_copy_kernel(u32x4* __restrict__ dst, 
   if (i < n) __builtin_nontemporal_store(src[i], dst + i);
 }
 
+// Placement probe: the memory traffic of a 16-bit D3Q19 pull step without its
+// arithmetic.  19 planes of n two-byte elements lie back to back in `a` and in
+// `b`; element i of plane q in b takes element i + o_q of plane q in a, with
+// o_q the neighbour offsets of a 512 x 512 x n/2^18 lattice (x fastest),
+// clamped to the plane.
+__constant__ int32_t k_probe_off[19] = {0,      1,       -1,      512,     -512,    262144,  -262144,
+                                        513,    -511,    511,     -513,    262145,  -262143, 262143,
+                                        -262145, 262656, -261632, 261632,  -262656};
+__global__ void __launch_bounds__(256) k_placement_probe(const uint16_t* __restrict__ a, uint16_t* __restrict__ b,
+                                                         uint32_t n) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int q = 0; q < 19; q++) {
+    const int64_t j = min<int64_t>(max<int64_t>((int64_t)i + k_probe_off[q], 0), (int64_t)n - 1);
+    b[(uint64_t)q * n + i] = a[(uint64_t)q * n + (uint64_t)j];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Bounce-back mask of the bulk kernel (wall_bounce_back, bounce_back.hpp:163-185):
 // bit 0 = the node is a wall; for a fluid node, bit i (1..18) = the upstream
@@ -723,6 +742,12 @@ hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s
   const uint64_t blocks = (n + 255) / 256;
   if (blocks == 0 || blocks > 0x7FFFFFFFull) return hipErrorInvalidValue;
   hipLaunchKernelGGL(hbm_copy_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (u32x4*)dst, (const u32x4*)src, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_placement_probe(const void* a, void* b, uint32_t n, hipStream_t s) {
+  if (n == 0 || n % 256 != 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_placement_probe, dim3(n / 256), dim3(256), 0, s, (const uint16_t*)a, (uint16_t*)b, n);
   return hipGetLastError();
 }
 

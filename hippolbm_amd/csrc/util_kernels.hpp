@@ -104,6 +104,8 @@ hipError_t launch_reshift(int storage, void* f, uint64_t stride, uint32_t N, Res
 hipError_t launch_macro(int storage, const void* f, uint64_t stride, uint32_t N, const uint8_t* flags, double* m0,
                         double* m1, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, uint64_t bytes, hipStream_t s);
+// 19 planes of n (a multiple of 256) two-byte elements at a and at b (38 n bytes each)
+hipError_t launch_placement_probe(const void* a, void* b, uint32_t n, hipStream_t s);
 int reduce_parts(uint32_t N);
 hipError_t launch_reduce(const double* m0, const double* m1, const uint8_t* flags, uint32_t N, double* part,
                          double* out, hipStream_t s);

@@ -315,8 +315,13 @@ typedef struct {
   double node_updates; /* lattice node updates performed so far                     */
   double gpu_ms;       /* device time of all chunks (HIP events)                    */
   uint64_t peak_bytes_in_use; /* the largest bytes_in_use after any admission        */
+  int64_t relocated;   /* arena pairs replaced by the placement check at create     */
 } hlbm_batch_info;
 
+/* Arenas of 8 GiB and more are placement-checked at create: the streaming
+ * pattern of a step is timed on them and on up to three fresh arena pairs (held
+ * together while the check runs, each when its size plus 4 GiB is free), and the
+ * fastest pair is kept (HLBM_PLACEMENT=0 turns it off). */
 int hlbm_batch_create(const hlbm_batch_desc* desc, hlbm_batch** out);
 int hlbm_batch_destroy(hlbm_batch* b);
 /* Queue a solve; fails with HLBM_ERR_NOMEM if it can never fit the budget. */

@@ -86,6 +86,27 @@ def test_alone_equals_in_the_mix(storage):
         assert np.array_equal(alone[1], res[ids[k]][1])
 
 
+@pytest.mark.parametrize("storage", ["f64", "f16"])
+def test_arena_placement_move_does_not_change_results(storage, monkeypatch):
+    """HLBM_PLACEMENT=force makes hlbm_batch_create take every fresh arena pair it
+    probes; the solves then run in the moved arenas with the same results."""
+    solves = mix(14)
+    outs = []
+    for mode in ("0", "force"):
+        monkeypatch.setenv("HLBM_PLACEMENT", mode)
+        b = Batch(storage=storage, budget_bytes=512 << 20, chunk_steps=8)
+        moved = b.info()["relocated"]
+        say(storage, "HLBM_PLACEMENT", mode, "arena pairs moved", moved)
+        assert moved == (0 if mode == "0" else 3)
+        ids = [b.submit(s) for s in solves]
+        res = run_all(b, ids)
+        outs.append([res[i] for i in ids])
+        b.close()
+    for a, c in zip(*outs):
+        for x, y in zip(a, c):
+            assert np.array_equal(x, y)
+
+
 def test_chunk_length_and_graph_do_not_change_results():
     solves = mix(12)
     outs = []
