@@ -98,6 +98,8 @@ struct Solve {
   bool retiring = false;      // retirement enqueued; DONE once `ev` has completed
   hipEvent_t ev = nullptr;
   int res_arena = 0;          // arena whose slice holds rho (N), u (3N) and the statistics (7), as doubles
+  bool staged = false;        // rho, u and the statistics also go to the pinned read-back pool at stage_off
+  uint64_t stage_off = 0;     // (doubles)
   std::vector<double> rho, u, f;
   std::vector<double> init_f; // initial populations (FP64 node layout), empty = init_rho * w
   double stats[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -131,6 +133,11 @@ struct hlbm_batch {
   size_t obs_cap = 0, obs_used[2] = {0, 0};
   uint64_t max_doms = 0, max_tiles = 0;
   double* f_tmp = nullptr;  // one FP64 population plane (keep_populations, initial populations)
+  // pinned read-back pool: a retiring solve's rho, u and statistics are copied
+  // here on the compute stream, so that polling and hlbm_batch_result never
+  // wait on a device queue (solves that find no room take the copy stream)
+  double* h_stage = nullptr;
+  FreeList fl_stage;
   double* d_part = nullptr; // simulation_state partial sums (retirement, one reduction at a time on `stream`)
   uint64_t max_nodes = 0;
   unsigned grid[2] = {8u, 8u};  // workgroups of the BGK and MRT launches (graph captured with these)
@@ -429,6 +436,11 @@ int hlbm_batch_create(const hlbm_batch_desc* desc, hlbm_batch** out) {
       return cleanup("events");
     b->h_ntiles_set[k][0] = b->h_ntiles_set[k][1] = 0;
   }
+  {
+    const uint64_t stage = std::min<uint64_t>((uint64_t)256 << 20, 4 * b->max_nodes + 7 * 64);  // doubles, <= 2 GiB
+    if (hipHostMalloc((void**)&b->h_stage, stage * sizeof(double)) != hipSuccess) return cleanup("read-back pool");
+    b->fl_stage.init(stage);
+  }
   if (hipStreamCreate(&b->stream) != hipSuccess || hipStreamCreate(&b->copy_stream) != hipSuccess ||
       hipStreamCreate(&b->side) != hipSuccess)
     return cleanup("stream");
@@ -471,6 +483,7 @@ int hlbm_batch_destroy(hlbm_batch* b) {
   for (int k = 0; k < 2; k++) {
     for (void* p : {(void*)b->h_doms_set[k], (void*)b->h_tiles_set[k], (void*)b->h_ntiles_set[k], (void*)b->h_obs[k]})
       if (p) (void)hipHostFree(p);
+    if (k == 0 && b->h_stage) (void)hipHostFree(b->h_stage);
     if (b->cev[k].start) (void)hipEventDestroy(b->cev[k].start);
     if (b->cev[k].end) (void)hipEventDestroy(b->cev[k].end);
   }
@@ -690,6 +703,13 @@ static int retire_one(hlbm_batch* b, Solve& s) {
     }
     if (hipStreamSynchronize(st) != hipSuccess) return bfail(HLBM_ERR_HIP, "sync");
   }
+  {
+    const uint64_t nd = 4 * (uint64_t)s.N + 7;
+    s.staged = b->fl_stage.alloc(nd, &s.stage_off);
+    if (s.staged && hipMemcpyAsync(b->h_stage + s.stage_off, rho, nd * sizeof(double), hipMemcpyDeviceToHost, st) !=
+                        hipSuccess)
+      return bfail(HLBM_ERR_HIP, "read-back");
+  }
   if (!s.ev && hipEventCreateWithFlags(&s.ev, hipEventDisableTiming) != hipSuccess) return bfail(HLBM_ERR_HIP, "event");
   if (hipEventRecord(s.ev, st) != hipSuccess) return bfail(HLBM_ERR_HIP, "event");
   s.retiring = true;
@@ -704,10 +724,14 @@ static int poll_retiring(hlbm_batch* b, bool wait, int* finished) {
     Solve& s = b->solves[b->retiring[k]];
     hipError_t q = (wait && k == 0) ? hipEventSynchronize(s.ev) : hipEventQuery(s.ev);
     if (q == hipSuccess) {
-      const double* dstat = (const double*)((const char*)b->arena[s.res_arena] + s.f_off * b->esize) + 4 * (size_t)s.N;
-      if (hipMemcpyAsync(s.stats, dstat, sizeof(s.stats), hipMemcpyDeviceToHost, b->copy_stream) != hipSuccess ||
-          hipStreamSynchronize(b->copy_stream) != hipSuccess)
-        return bfail(HLBM_ERR_HIP, "statistics copy");
+      if (s.staged) {
+        std::memcpy(s.stats, b->h_stage + s.stage_off + 4 * (size_t)s.N, sizeof(s.stats));
+      } else {
+        const double* dstat = (const double*)((const char*)b->arena[s.res_arena] + s.f_off * b->esize) + 4 * (size_t)s.N;
+        if (hipMemcpyAsync(s.stats, dstat, sizeof(s.stats), hipMemcpyDeviceToHost, b->copy_stream) != hipSuccess ||
+            hipStreamSynchronize(b->copy_stream) != hipSuccess)
+          return bfail(HLBM_ERR_HIP, "statistics copy");
+      }
       bool finite = true;
       for (double v : s.stats) finite = finite && std::isfinite(v);
       s.retiring = false;
@@ -910,12 +934,20 @@ int hlbm_batch_result(hlbm_batch* b, int64_t id, double* rho, double* u, double*
   Solve& s = it->second;
   if (s.state != HLBM_SOLVE_DONE && s.state != HLBM_SOLVE_FAILED) return bfail(HLBM_ERR_ARG, "solve not finished");
   if (f && !s.desc.keep_populations) return bfail(HLBM_ERR_ARG, "populations were not kept (keep_populations = 0)");
-  // the read-out waits for nothing on the main stream: the fields are final
-  const double* dres = (const double*)((const char*)b->arena[s.res_arena] + s.f_off * b->esize);
-  if ((rho && hipMemcpyAsync(rho, dres, (size_t)s.N * 8, hipMemcpyDeviceToHost, b->copy_stream) != hipSuccess) ||
-      (u && hipMemcpyAsync(u, dres + s.N, 3 * (size_t)s.N * 8, hipMemcpyDeviceToHost, b->copy_stream) != hipSuccess) ||
-      hipStreamSynchronize(b->copy_stream) != hipSuccess)
-    return bfail(HLBM_ERR_HIP, "result copy");
+  if (s.staged) {
+    // the retirement's read-back has completed (the solve is finished)
+    const double* h = b->h_stage + s.stage_off;
+    if (rho) std::memcpy(rho, h, (size_t)s.N * 8);
+    if (u) std::memcpy(u, h + s.N, 3 * (size_t)s.N * 8);
+    b->fl_stage.release(s.stage_off, 4 * (uint64_t)s.N + 7);
+  } else {
+    // the read-out waits for nothing on the main stream: the fields are final
+    const double* dres = (const double*)((const char*)b->arena[s.res_arena] + s.f_off * b->esize);
+    if ((rho && hipMemcpyAsync(rho, dres, (size_t)s.N * 8, hipMemcpyDeviceToHost, b->copy_stream) != hipSuccess) ||
+        (u && hipMemcpyAsync(u, dres + s.N, 3 * (size_t)s.N * 8, hipMemcpyDeviceToHost, b->copy_stream) != hipSuccess) ||
+        hipStreamSynchronize(b->copy_stream) != hipSuccess)
+      return bfail(HLBM_ERR_HIP, "result copy");
+  }
   if (f) std::memcpy(f, s.f.data(), s.f.size() * 8);
   b->fl_pop.release(s.f_off, s.f_len);
   b->fl_flag.release(s.flag_off, s.flag_len);
