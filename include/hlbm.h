@@ -321,7 +321,9 @@ typedef struct {
 /* Arenas of 8 GiB and more are placement-checked at create: the streaming
  * pattern of a step is timed on them and on up to three fresh arena pairs (held
  * together while the check runs, each when its size plus 4 GiB is free), and the
- * fastest pair is kept (HLBM_PLACEMENT=0 turns it off). */
+ * fastest pair is kept (HLBM_PLACEMENT=0 turns it off).  The batch also holds a
+ * pinned host pool of min(2 GiB, 32 bytes per budgeted node) into which a
+ * retiring solve's rho, u and statistics are copied on the compute stream. */
 int hlbm_batch_create(const hlbm_batch_desc* desc, hlbm_batch** out);
 int hlbm_batch_destroy(hlbm_batch* b);
 /* Queue a solve; fails with HLBM_ERR_NOMEM if it can never fit the budget. */
