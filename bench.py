@@ -359,6 +359,8 @@ def main():
     ap.add_argument("--dtypes", default="f64,f32,f16,bf16")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", default=None, help="run a single storage type (profiling)")
+    ap.add_argument("--pattern", choices=("ab", "aa"), default="ab",
+                    help="with --only: A/B buffers or the in-place AA pattern")
     ap.add_argument("--copy", action="store_true", help="run only the streaming-copy reference kernel (profiling)")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-cavity and served-stream legs")
     ap.add_argument("--batch-only", nargs="?", const="both", choices=("both", "batched", "served"), default=None,
@@ -393,8 +395,8 @@ def main():
         print(json.dumps({"precision_sweep": precision_sweep(args.precision)}), flush=True)
         return 0
     if args.only:
-        r = measure(args.size, args.only, args.steps, args.warmup, tune=not args.no_tune)
-        print(json.dumps({"storage": args.only, **r}), flush=True)
+        r = measure(args.size, args.only, args.steps, args.warmup, tune=not args.no_tune, pattern=args.pattern)
+        print(json.dumps({"storage": args.only, "pattern": args.pattern, **r}), flush=True)
         return 0
     if args.batch_only:
         g = args.batch_launch == "graph"
