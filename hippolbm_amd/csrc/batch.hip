@@ -27,6 +27,7 @@
 
 #include "../../include/hlbm.h"
 #include "common.hpp"
+#include "free_list.hpp"
 #include "launch.hpp"
 #include "util_kernels.hpp"
 
@@ -41,47 +42,6 @@ double now_s() {
   static const steady_clock::time_point t0 = steady_clock::now();
   return duration<double>(steady_clock::now() - t0).count();
 }
-
-// first-fit allocator over [0, size) in abstract units
-struct FreeList {
-  std::map<uint64_t, uint64_t> free;  // offset -> length
-  uint64_t size = 0, used = 0;
-  void init(uint64_t n) {
-    size = n;
-    used = 0;
-    free.clear();
-    if (n) free[0] = n;
-  }
-  bool alloc(uint64_t n, uint64_t* off) {
-    for (auto it = free.begin(); it != free.end(); ++it) {
-      if (it->second >= n) {
-        *off = it->first;
-        const uint64_t rest = it->second - n, roff = it->first + n;
-        free.erase(it);
-        if (rest) free[roff] = rest;
-        used += n;
-        return true;
-      }
-    }
-    return false;
-  }
-  void release(uint64_t off, uint64_t n) {
-    used -= n;
-    auto it = free.emplace(off, n).first;
-    auto nx = std::next(it);
-    if (nx != free.end() && it->first + it->second == nx->first) {
-      it->second += nx->second;
-      free.erase(nx);
-    }
-    if (it != free.begin()) {
-      auto pv = std::prev(it);
-      if (pv->first + pv->second == it->first) {
-        pv->second += it->second;
-        free.erase(it);
-      }
-    }
-  }
-};
 
 struct Solve {
   int64_t id = 0;
